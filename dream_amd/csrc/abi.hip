@@ -1,0 +1,343 @@
+// abi.hip -- the extern "C" boundary of libdream_hip.so (include/dream_hip.h).
+//
+// Every entry point converts C++ exceptions into DREAM_HIP_E* codes and keeps
+// the message for dream_hip_last_error(); no C++ type crosses the boundary.
+
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "das.hpp"
+#include "sir.hpp"
+
+namespace dream {
+
+namespace {
+thread_local std::string g_last_error;
+std::atomic<bool> g_running(true);
+thread_local hipStream_t g_stream = nullptr;
+} // namespace
+
+void hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+  char buf[512];
+  std::snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int) e,
+                hipGetErrorString(e), what, file, line);
+  throw Error(e == hipErrorOutOfMemory ? DREAM_HIP_ENOMEM : DREAM_HIP_ERUNTIME, buf);
+}
+
+bool cancelled() { return !g_running.load(std::memory_order_relaxed); }
+
+hipStream_t lib_stream()
+{
+  if (!g_stream) DREAM_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  return g_stream;
+}
+
+template <class F>
+int guarded(F f)
+{
+  try {
+    g_last_error.clear();
+    return f();
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return DREAM_HIP_ENOMEM;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return DREAM_HIP_ERUNTIME;
+  }
+}
+
+// Device copy of a host array of doubles (freed by the owner).
+struct DevBuf {
+  void *p = nullptr;
+  DevBuf() = default;
+  DevBuf(const void *src, size_t bytes, hipStream_t s)
+  {
+    if (!bytes) return;
+    if (hipMalloc(&p, bytes) != hipSuccess) throw Error(DREAM_HIP_ENOMEM, "device allocation failed");
+    DREAM_HIP_CHECK(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s));
+  }
+  explicit DevBuf(size_t bytes)
+  {
+    if (bytes && hipMalloc(&p, bytes) != hipSuccess)
+      throw Error(DREAM_HIP_ENOMEM, "device allocation failed");
+  }
+  ~DevBuf() { if (p) (void) hipFree(p); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+};
+
+SirProblem make_problem(const double *Ro, int64_t No, double a, double b, double dx, double dy,
+                        double dt, int64_t nt, int delay_type, const double *delay, double v,
+                        double cp, int64_t L, const double *G, int foc_met, const double *focal,
+                        int steer_met, double theta, double phi, const double *apod,
+                        int do_apod, int apod_met, double apod_par, void *h, int err_level)
+{
+  require(L >= 1 && G, "dream_arr_rect: need a grid G with at least one element");
+  require(foc_met >= 0 && foc_met <= 7, "dream_arr_rect: unknown focusing method");
+  require(steer_met >= 0 && steer_met <= 3, "dream_arr_rect: unknown steering method");
+  require(!do_apod || (apod_met >= 0 && apod_met <= 7), "dream_arr_rect: unknown apodization");
+  require(err_level >= 0 && err_level <= 4, "dream_arr_rect: unknown error level");
+  SirProblem p;
+  p.Ro = Ro;
+  p.No = No;
+  p.a = a; p.b = b; p.dx = dx; p.dy = dy; p.dt = dt; p.nt = nt;
+  p.delay_type = delay_type;
+  p.delay = delay;
+  p.v = v;
+  p.cp = cp;
+  p.G.assign(G, G + 3 * L);
+  p.foc_met = foc_met;
+  if (foc_met == 7) {
+    require(focal != nullptr, "dream_arr_rect: 'ud' focusing needs L focal delays");
+    p.focal.assign(focal, focal + L);
+  } else {
+    p.focal.assign(1, focal ? focal[0] : 0.0);
+  }
+  p.steer_met = steer_met;
+  p.theta = theta;
+  p.phi = phi;
+  p.do_apod = do_apod;
+  p.apod_met = apod_met;
+  p.apod_par = apod_par;
+  if (do_apod && apod_met == 7) {
+    require(apod != nullptr, "dream_arr_rect: 'ud' apodization needs L weights");
+    p.apod.assign(apod, apod + L);
+  }
+  p.err_level = err_level;
+  p.h = h;
+  return p;
+}
+
+} // namespace dream
+
+using namespace dream;
+
+extern "C" {
+
+const char *dream_hip_last_error(void) { return g_last_error.c_str(); }
+
+const char *dream_hip_version(void) { return "dream_hip 0.1 (gfx950)"; }
+
+int dream_hip_device_count(void)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void dream_hip_abort(int) { g_running.store(false); }
+void dream_hip_set_running(void) { g_running.store(true); }
+int dream_hip_is_running(void) { return g_running.load() ? 1 : 0; }
+
+// ------------------------------------------------------------------ SIR
+
+int dream_hip_arr_rect_dev(const double *Ro, int64_t No, double a, double b, double dx,
+                           double dy, double dt, int64_t nt, int delay_type,
+                           const double *delay, double v, double cp, int64_t num_elements,
+                           const double *G, int foc_met, const double *focal, int steer_met,
+                           double theta, double phi, const double *apod, int do_apod,
+                           int apod_met, double apod_par, void *h, int out_dtype, int err_level,
+                           int *sir_err, int64_t *oob_count, void *stream)
+{
+  return guarded([&]() {
+    hipStream_t s = stream ? (hipStream_t) stream : lib_stream();
+    std::vector<SirProblem> probs;
+    probs.push_back(make_problem(Ro, No, a, b, dx, dy, dt, nt, delay_type, delay, v, cp,
+                                 num_elements, G, foc_met, focal, steer_met, theta, phi, apod,
+                                 do_apod, apod_met, apod_par, h, err_level));
+    SirPlan plan(probs, out_dtype);
+    plan.run(s, false);
+    int32_t code = 0;
+    int64_t oob = 0;
+    plan.status(&code, &oob);   // synchronises `s`
+    if (sir_err) *sir_err = code;
+    if (oob_count) *oob_count = oob;
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_arr_rect(double alpha, const double *Ro, int64_t No, double a, double b,
+                       double dx, double dy, double dt, int64_t nt, int delay_type,
+                       const double *delay, double v, double cp, int64_t num_elements,
+                       const double *G, int foc_met, const double *focal, int steer_met,
+                       double theta, double phi, const double *apod, int do_apod, int apod_met,
+                       double apod_par, double *h, int err_level, int *sir_err)
+{
+  return guarded([&]() {
+    if (alpha > 2.220446049250313e-16)
+      throw Error(DREAM_HIP_EUNSUPPORTED, "dream_arr_rect: attenuation (alpha > 0) is not on the GPU path");
+    require(No >= 0 && nt >= 1, "dream_arr_rect: need No >= 0 and nt >= 1");
+    require(No == 0 || (Ro && delay && h), "dream_arr_rect: null argument");
+    hipStream_t s = lib_stream();
+    const int64_t nd = delay_type ? No : 1;
+    DevBuf dRo(Ro, (size_t) No * 3 * 8, s), dD(delay, (size_t) nd * 8, s);
+    DevBuf dH((size_t) No * nt * 8);
+    int code = 0;
+    int rc = dream_hip_arr_rect_dev((const double *) dRo.p, No, a, b, dx, dy, dt, nt, delay_type,
+                                    (const double *) dD.p, v, cp, num_elements, G, foc_met, focal,
+                                    steer_met, theta, phi, apod, do_apod, apod_met, apod_par, dH.p,
+                                    DREAM_HIP_F64, err_level, &code, nullptr, s);
+    if (rc != DREAM_HIP_OK) throw Error(rc, g_last_error);
+    if (No) DREAM_HIP_CHECK(hipMemcpy(h, dH.p, (size_t) No * nt * 8, hipMemcpyDeviceToHost));
+    if (sir_err) *sir_err = code;
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_rect(double alpha, const double *Ro, int64_t No, double a, double b, double dx,
+                   double dy, double dt, int64_t nt, int delay_type, const double *delay,
+                   double v, double cp, double *h, int err_level, int *sir_err)
+{
+  const double G[3] = {0.0, 0.0, 0.0};
+  const double focal = 0.0;
+  return dream_hip_arr_rect(alpha, Ro, No, a, b, dx, dy, dt, nt, delay_type, delay, v, cp, 1, G,
+                            0, &focal, 0, 0.0, 0.0, nullptr, 0, 0, 0.0, h, err_level, sir_err);
+}
+
+int dream_hip_rect_indices(double xo, double yo, double zo, double a, double b, double dx,
+                           double dy, double dt, double delay, double cp, int64_t *it_out,
+                           int64_t cap, int64_t *count)
+{
+  return guarded([&]() {
+    require(it_out || cap == 0, "rect_indices: null output");
+    require(count != nullptr, "rect_indices: null count");
+    rect_indices(xo, yo, zo, a, b, dx, dy, dt, delay, cp, it_out, cap, count);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_sir_plan_create(const DreamSirProblem *problems, int32_t n, int out_dtype,
+                              void **plan)
+{
+  return guarded([&]() {
+    require(plan != nullptr && (problems || n == 0) && n >= 0, "sir_plan_create: bad arguments");
+    std::vector<SirProblem> probs;
+    probs.reserve((size_t) n);
+    for (int32_t i = 0; i < n; i++) {
+      const DreamSirProblem &q = problems[i];
+      probs.push_back(make_problem(q.Ro, q.No, q.a, q.b, q.dx, q.dy, q.dt, q.nt, q.delay_type,
+                                   q.delay, q.v, q.cp, q.num_elements, q.G, q.foc_met, q.focal,
+                                   q.steer_met, q.theta, q.phi, q.apod, q.do_apod, q.apod_met,
+                                   q.apod_par, q.h, q.err_level));
+    }
+    *plan = new SirPlan(probs, out_dtype);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_sir_plan_run(void *plan, void *stream, int use_graph)
+{
+  return guarded([&]() {
+    require(plan != nullptr, "sir_plan_run: null plan");
+    static_cast<SirPlan *>(plan)->run(stream ? (hipStream_t) stream : lib_stream(), use_graph != 0);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_sir_plan_status(void *plan, int32_t *sir_err, int64_t *oob_count)
+{
+  return guarded([&]() {
+    require(plan != nullptr, "sir_plan_status: null plan");
+    static_cast<SirPlan *>(plan)->status(sir_err, oob_count);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_sir_plan_destroy(void *plan)
+{
+  return guarded([&]() {
+    delete static_cast<SirPlan *>(plan);
+    return DREAM_HIP_OK;
+  });
+}
+
+// ------------------------------------------------------------------ DAS
+
+int dream_hip_das_create(int das_type, int64_t a_scan_len, int64_t No, int64_t num_t_elements,
+                         int64_t num_r_elements, int precision, void **handle)
+{
+  return guarded([&]() {
+    require(handle != nullptr, "das_create: null handle pointer");
+    *handle = new DasHandle(das_type, a_scan_len, No, num_t_elements, num_r_elements, precision);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_destroy(void *handle)
+{
+  return guarded([&]() {
+    delete static_cast<DasHandle *>(handle);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_setup_has_changed(void *handle, int das_type, int64_t a_scan_len, int64_t No,
+                                    int64_t num_t_elements, int64_t num_r_elements,
+                                    int precision)
+{
+  if (!handle) return 1;
+  return static_cast<DasHandle *>(handle)->changed(das_type, a_scan_len, No, num_t_elements,
+                                                   num_r_elements, precision)
+             ? 1
+             : 0;
+}
+
+int dream_hip_das(void *handle, const void *Y, const void *Ro, const void *Gt, const void *Gr,
+                  double dt, int delay_type, const void *delay, double cp, void *Im,
+                  int err_level, int *sir_err)
+{
+  return guarded([&]() {
+    require(handle != nullptr, "das: null handle");
+    require(err_level >= 0 && err_level <= 4, "das: unknown error level");
+    int code = static_cast<DasHandle *>(handle)->run_host(Y, Ro, Gt, Gr, dt, delay_type, delay, cp,
+                                                          Im, err_level);
+    if (sir_err) *sir_err = code;
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_dev(void *handle, const void *Y, const void *Ro, const void *Gt, const void *Gr,
+                      double dt, int delay_type, const void *delay, double cp,
+                      const void *apod_t, const void *apod_r, int interp, int64_t n_frames,
+                      int64_t y_frame_stride, int64_t im_frame_stride, void *Im, int err_level,
+                      int *sir_err, void *stream)
+{
+  return guarded([&]() {
+    require(handle != nullptr, "das: null handle");
+    require(err_level >= 0 && err_level <= 4, "das: unknown error level");
+    hipStream_t s = stream ? (hipStream_t) stream : lib_stream();
+    int code = static_cast<DasHandle *>(handle)->run_dev(
+        Y, Ro, Gt, Gr, dt, delay_type, delay, cp, apod_t, apod_r, interp, n_frames,
+        y_frame_stride, im_frame_stride, Im, err_level, sir_err != nullptr, s);
+    if (sir_err) *sir_err = code;
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const void *Gt,
+                              const void *Gr, double dt, double delay, double cp,
+                              const void *apod_t, const void *apod_r, int interp,
+                              int64_t n_frames, int64_t y_frame_stride, int64_t im_frame_stride,
+                              const void *g, void *gY, void *g_apod_t, void *g_apod_r,
+                              void *g_cp, void *stream)
+{
+  return guarded([&]() {
+    require(handle != nullptr, "das adjoint: null handle");
+    hipStream_t s = stream ? (hipStream_t) stream : lib_stream();
+    static_cast<DasHandle *>(handle)->adjoint_dev(Y, Ro, Gt, Gr, dt, delay, cp, apod_t, apod_r,
+                                                  interp, n_frames, y_frame_stride,
+                                                  im_frame_stride, g, gY, g_apod_t, g_apod_r,
+                                                  g_cp, s);
+    return DREAM_HIP_OK;
+  });
+}
+
+} // extern "C"

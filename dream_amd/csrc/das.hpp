@@ -1,0 +1,73 @@
+// das.hpp -- host side of the delay-and-sum (das) kernels.
+#pragma once
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace dream {
+
+// Arguments of one DAS launch (all device pointers).
+struct DasArgs {
+  const void *Y;  int64_t K;
+  const void *Ro;  int64_t No;
+  const void *Gt;  int32_t Lt;
+  const void *Gr;  int32_t Lr;
+  const void *delay;  int32_t delay_multi;
+  double fs_khz, inv_cp, delay_ms, cp;   // in the geometry type, passed as double
+  const void *apod_t, *apod_r;
+  int64_t n_frames, y_stride, im_stride;
+  void *Im;
+  Status *status;
+  // Adjoint-only.
+  const void *g;  void *gY;  void *z_t;  void *z_r;  void *z_cp;
+};
+
+class DasHandle {
+public:
+  DasHandle(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, int precision);
+  ~DasHandle();
+  bool changed(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, int precision) const;
+
+  // Host-pointer call (DAS<T>::das semantics); returns the SIRError code.
+  int run_host(const void *Y, const void *Ro, const void *Gt, const void *Gr, double dt,
+               int delay_type, const void *delay, double cp, void *Im, int err_level);
+
+  // Device-pointer forward; the status code is returned only when
+  // `want_status` (then the call synchronises).
+  int run_dev(const void *Y, const void *Ro, const void *Gt, const void *Gr, double dt,
+              int delay_type, const void *delay, double cp, const void *apod_t,
+              const void *apod_r, int interp, int64_t n_frames, int64_t y_stride,
+              int64_t im_stride, void *Im, int err_level, bool want_status, hipStream_t s);
+
+  void adjoint_dev(const void *Y, const void *Ro, const void *Gt, const void *Gr, double dt,
+                   double delay, double cp, const void *apod_t, const void *apod_r, int interp,
+                   int64_t n_frames, int64_t y_stride, int64_t im_stride, const void *g,
+                   void *gY, void *g_apod_t, void *g_apod_r, void *g_cp, hipStream_t s);
+
+  int type() const { return type_; }
+  int precision() const { return prec_; }
+
+private:
+  void fill_args(DasArgs &a, const void *Y, const void *Ro, const void *Gt, const void *Gr,
+                 double dt, int delay_type, const void *delay, double cp, const void *apod_t,
+                 const void *apod_r, int64_t n_frames, int64_t y_stride, int64_t im_stride,
+                 void *Im) const;
+  int type_;           // effective type (SAFT when Lr == 0)
+  int64_t K_, No_, Lt_, Lr_;
+  int prec_;
+  Status *d_status_ = nullptr;
+  Status *h_status_ = nullptr;   // pinned
+  // Device copies for the host-pointer path (allocated on first use).
+  void *d_buf_ = nullptr;
+  size_t d_buf_bytes_ = 0;
+  void *d_z_ = nullptr;          // adjoint channel accumulators
+  size_t d_z_bytes_ = 0;
+};
+
+void launch_das_forward(const DasArgs &a, int type, int prec, int interp, bool apod,
+                        hipStream_t s);
+void launch_das_adjoint(const DasArgs &a, int type, int prec, int interp, bool apod,
+                        bool want_gy, hipStream_t s);
+
+} // namespace dream

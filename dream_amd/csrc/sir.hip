@@ -1,0 +1,622 @@
+// sir.hip -- discrete-representation spatial impulse response of arrays of
+// rectangular elements (dream_arr_rect, dreamrect) on gfx950.
+//
+// Reference algorithm: src/dreamrect.cc:274-335 (the DR sum) driven by
+// src/dream_arr_rect.cc:192-244 (per-element delay and weight).  For every
+// observation point, element and sub-element centre (xs, ys):
+//   r  = sqrt((rx*rx + ry*ry) + rz*rz)
+//   ai = ((w v) ds) / (2 pi r) / dt * 1000
+//   it = rint((((r*1000)/cp) - d)/dt),  d = (delay - foc) - steer
+//   h[it] += ai when 0 <= it < nt.
+//
+// Work decomposition.  A work item is one observation point and a chunk of
+// its (element, row) pairs; the whole batch of problems is one list of work
+// items in cost order, served either one work-group per item or by a
+// persistent grid pulling items from an atomic counter.  Each 256-thread
+// work-group accumulates into an LDS histogram of nt bins and writes the
+// column once (or adds its bins into a zeroed column when a point is split
+// over several items).  Each lane walks one row of sub-elements, keeping a
+// run-length (bin, sum) in registers and touching LDS only when the bin
+// changes.  Rows are handed to lanes in a strided order so the 64 lanes of
+// a wave see rows spread over the whole element and hit different bins.
+//
+// Sample index.  The index must equal the reference's FP64 sequence bit for
+// bit.  A fast estimate u = r*c1 - d/dt (r from the hardware rsqrt and two
+// Goldschmidt steps, c1 = 1000/(cp dt)) differs from the reference's
+// pre-rounding value by far less than 2^-40 (|u| + |d/dt| + 1); when u's
+// fractional part is farther than that from 1/2, rint of both agree.  Only
+// the other contributions (about one in 10^6) take the exact IEEE path:
+// correctly rounded sqrt, two correctly rounded divisions and rint.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "sir.hpp"
+
+namespace dream {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr double kBoundScale = 0x1p-40;
+
+// Sample index (as a double) and 1/r for squared distance s; see the file
+// comment for the error argument.
+__device__ __forceinline__ double sir_index(double s, double c1, double c2, double cbound,
+                                            double cp, double d, double dt, double &rinv)
+{
+  double r;
+  sqrt_rsqrt(s, r, rinv);
+  const double u = __builtin_fma(r, c1, -c2);
+  const double fl = __builtin_floor(u);
+  const double fr = u - fl;
+  const double bound = __builtin_fma(__builtin_fabs(u), kBoundScale, cbound);
+  double ru = fl + (fr > 0.5 ? 1.0 : 0.0);
+  if (!(s > 0x1p-900 && __builtin_fabs(fr - 0.5) > bound)) {
+    const double re = __builtin_sqrt(s);
+    const double t = (re * 1000.0) / cp;
+    ru = __builtin_rint((t - d) / dt);
+    rinv = 1.0 / re;
+  }
+  return ru;
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_add(T *p, T v)
+{
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <typename T>
+__device__ __forceinline__ void global_add(T *p, T v)
+{
+  unsafeAtomicAdd(p, v);
+}
+
+__device__ __forceinline__ unsigned wave_sum(unsigned v)
+{
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ int find_problem(const SirDev *P, int n, int64_t item)
+{
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (P[mid].item_begin <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One work item: observation point n of problem P, (element,row) chunk c.
+template <typename T>
+__device__ void sir_item(const SirDev &P, int64_t local, T *hist)
+{
+  const int64_t n = local / P.chunks;
+  const int c = (int) (local - n * P.chunks);
+  const double xo = P.Ro[n], yo = P.Ro[n + P.No], zo = P.Ro[n + 2 * P.No];
+  const double dlay = P.delay_multi ? P.delay[n] : P.delay[0];
+  const int ny = P.ny, nx = P.nx, L = P.L;
+  const int Q = L * ny;
+  const int q0 = (int) ((int64_t) c * Q / P.chunks);
+  const int q1 = (int) ((int64_t) (c + 1) * Q / P.chunks);
+  const int64_t nt = P.nt;
+  const double ntd = (double) nt;
+  T *out = reinterpret_cast<T *>(P.h) + n * nt;
+  const bool use_lds = P.mode != SIR_GLOBAL;
+
+  if (use_lds) {
+    for (int i = threadIdx.x; i < nt; i += kThreads) hist[i] = T(0);
+    __syncthreads();
+  }
+
+  unsigned oob = 0;
+  bool oob_last = false;
+  int cur = -1;
+  T acc = T(0);
+  const double *el = P.el;
+
+  for (int q = q0 + (int) threadIdx.x; q < q1; q += kThreads) {
+    const int e = q / ny;
+    const int j = q - e * ny;
+    const int row = (int) (((int64_t) j * P.row_stride) % ny);
+    const double xe = xo - el[e];
+    const double ye = yo - el[L + e];
+    const double ze = zo - el[2 * L + e];
+    const double d = (dlay - el[3 * L + e]) - el[4 * L + e];
+    const double ry = ye - P.ys[row];
+    const double ryy = ry * ry;
+    const double rzz = ze * ze;
+    const double c2 = d * P.inv_dt;
+    const double cbound = kBoundScale * (__builtin_fabs(c2) + 1.0);
+    const double amp = el[5 * L + e] * P.amp;
+    const bool last = (e == L - 1);
+    for (int i = 0; i < nx; i++) {
+      const double rx = xe - P.xs[i];
+      const double s = (rx * rx + ryy) + rzz;
+      double rinv;
+      const double ru = sir_index(s, P.c1, c2, cbound, P.cp, d, P.dt, rinv);
+      const T ai = (T) (amp * rinv);
+      if (ru >= 0.0 && ru < ntd) {
+        const int bin = (int) ru;
+        if (bin != cur) {
+          if (cur >= 0) {
+            if (use_lds) lds_add(hist + cur, acc); else global_add(out + cur, acc);
+          }
+          cur = bin;
+          acc = ai;
+        } else {
+          acc += ai;
+        }
+      } else {
+        oob++;
+        oob_last |= last;
+      }
+    }
+  }
+  if (cur >= 0) {
+    if (use_lds) lds_add(hist + cur, acc); else global_add(out + cur, acc);
+  }
+
+  // Status: wave-reduced, one atomic per wave.
+  const unsigned wsum = wave_sum(oob);
+  const bool any_last = __any(oob_last);
+  if ((threadIdx.x & 63) == 0 && wsum) {
+    atomicOr(&P.status->any_oob, 1u);
+    if (any_last) atomicOr(&P.status->last_elem_oob, 1u);
+    atomicAdd(&P.status->oob, (unsigned long long) wsum);
+  }
+
+  if (use_lds) {
+    __syncthreads();
+    if (P.mode == SIR_LDS_STORE) {
+      for (int i = threadIdx.x; i < nt; i += kThreads) out[i] = hist[i];
+    } else {
+      for (int i = threadIdx.x; i < nt; i += kThreads) {
+        const T v = hist[i];
+        if (v != T(0)) global_add(out + i, v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+sir_kernel(const SirDev *__restrict__ probs, int n_probs, int64_t n_items,
+           unsigned long long *counter)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *hist = reinterpret_cast<T *>(smem);
+  if (counter == nullptr) {
+    const int64_t item = blockIdx.x;
+    if (item >= n_items) return;
+    const int p = find_problem(probs, n_probs, item);
+    sir_item<T>(probs[p], item - probs[p].item_begin, hist);
+    return;
+  }
+  // Persistent: pull items from counter[0]; the last work-group to leave
+  // resets both counters for the next launch.
+  __shared__ int64_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int64_t) atomicAdd(&counter[0], 1ull);
+    __syncthreads();
+    const int64_t item = s_item;
+    __syncthreads();
+    if (item >= n_items) break;
+    const int p = find_problem(probs, n_probs, item);
+    sir_item<T>(probs[p], item - probs[p].item_begin, hist);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned long long done = atomicAdd(&counter[1], 1ull);
+    if (done == gridDim.x - 1) {
+      atomicExch(&counter[0], 0ull);
+      atomicExch(&counter[1], 0ull);
+    }
+  }
+}
+
+// Zero `n` elements of `bytes_per`-byte words (graph-safe replacement for a
+// memset node).
+__global__ void fill_zero_kernel(unsigned int *p, int64_t nwords)
+{
+  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nwords;
+       i += (int64_t) gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+__global__ void rect_indices_kernel(double xo, double yo, double zo, const double *xs, int nx,
+                                    const double *ys, int ny, double cp, double dt,
+                                    double delay, int64_t *it_out, int64_t cap)
+{
+  const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t) nx * ny || q >= cap) return;
+  const int row = (int) (q / nx), col = (int) (q - (int64_t) row * nx);
+  const double ry = yo - ys[row];
+  const double rx = xo - xs[col];
+  const double s = (rx * rx + ry * ry) + zo * zo;
+  const double c1 = 1000.0 / (cp * dt);
+  const double c2 = delay * (1.0 / dt);
+  const double cbound = kBoundScale * (__builtin_fabs(c2) + 1.0);
+  double rinv;
+  it_out[q] = (int64_t) sir_index(s, c1, c2, cbound, cp, delay, dt, rinv);
+}
+
+int gcd_int(int a, int b)
+{
+  while (b) { int t = a % b; a = b; b = t; }
+  return a;
+}
+
+int device_cus()
+{
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    DREAM_HIP_CHECK(hipGetDevice(&dev));
+    DREAM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
+
+void launch_fill_zero(void *p, size_t bytes, hipStream_t s)
+{
+  if (!bytes) return;
+  const int64_t words = (int64_t) (bytes / 4);
+  const int blocks = (int) std::min<int64_t>((words + 255) / 256, 4096);
+  fill_zero_kernel<<<blocks, 256, 0, s>>>(reinterpret_cast<unsigned int *>(p), words);
+  DREAM_HIP_CHECK(hipGetLastError());
+}
+
+} // namespace
+
+// ------------------------------------------------------------------ host helpers
+
+std::vector<double> subgrid(double w, double d)
+{
+  std::vector<double> v;
+  const double smax = w / 2.0;
+  double s = -w / 2.0 + d / 2.0;
+  while (s <= smax) {
+    v.push_back(s);
+    s += d;
+  }
+  return v;
+}
+
+// Array helpers (src/arr_functions.cc:60-262), evaluated per element in the
+// reference's operation order so the delays -- and hence the sample
+// indices -- are bit-identical.
+namespace {
+
+void aperture_extent(const double *gx, const double *gy, int64_t L, double &x_max,
+                     double &y_max, double &r_max)
+{
+  x_max = std::fabs(gx[0]);
+  for (int64_t i = 1; i < L; i++) if (std::fabs(gx[i]) > x_max) x_max = std::fabs(gx[i]);
+  y_max = std::fabs(gy[0]);
+  for (int64_t i = 1; i < L; i++) if (std::fabs(gy[i]) > y_max) y_max = std::fabs(gy[i]);
+  r_max = std::sqrt(x_max * x_max + y_max * y_max);
+}
+
+double focus_delay(int met, double focal, double gx, double gy, double x_max, double y_max,
+                   double r_max, double cp)
+{
+  switch (met) {
+  case 1: { // x
+    double outer = std::sqrt(x_max * x_max + focal * focal);
+    return (outer - std::sqrt(gx * gx + focal * focal)) * 1.0e3 / cp;
+  }
+  case 3: { // y
+    double outer = std::sqrt(y_max * y_max + focal * focal);
+    return (outer - std::sqrt(gy * gy + focal * focal)) * 1.0e3 / cp;
+  }
+  case 5: { // xy
+    double outer = std::sqrt(r_max * r_max + focal * focal);
+    return (outer - std::sqrt(gx * gx + gy * gy + focal * focal)) * 1.0e3 / cp;
+  }
+  case 6: { // x+y
+    double outer = std::sqrt(r_max * r_max + focal * focal);
+    double px = std::sqrt(gx * gx + focal * focal);
+    double py = std::sqrt(gy * gy + focal * focal);
+    return (outer - (px + py)) * 1.0e3 / cp;
+  }
+  case 7:   // ud
+    return focal;
+  default:  // none, negx, negy
+    return 0.0;
+  }
+}
+
+double steer_delay(int met, double theta, double phi, double gx, double gy, double x_max,
+                   double y_max, double cp)
+{
+  const double deg2rad = M_PI / (double) 180.0;
+  auto leg = [&](double ang, double ext, double g) {
+    double sn = std::sin(ang * deg2rad);
+    double e = ext * sn;
+    return (e + g * sn) * 1.0e3 / cp;
+  };
+  switch (met) {
+  case 1: return leg(theta, x_max, gx);
+  case 2: return leg(phi, y_max, gy);
+  case 3: {
+    double sx = leg(theta, x_max, gx);
+    double sy = leg(phi, y_max, gy);
+    return sx + sy;
+  }
+  default: return 0.0;
+  }
+}
+
+double apod_weight(int met, int64_t n, const double *ud, double gx, double gy, double r_max,
+                   double par)
+{
+  const double r = std::sqrt(gx * gx + gy * gy);
+  switch (met) {
+  case 7: return ud[n];
+  case 0: return 1.0 - std::fabs(r) / r_max;
+  case 1: return std::exp(-(par * r * r) / (r_max * r_max));
+  case 2: return par + std::cos(r * M_PI / r_max);
+  case 3: return 0.5 * (1.0 - std::cos((r + r_max) * M_PI / r_max));
+  case 4: {
+    const double a0 = 25.0 / 46.0;
+    return a0 - (1.0 - a0) * std::cos((r + r_max) * M_PI / r_max);
+  }
+  case 5: return 1.0 - r * r / (r_max * r_max);
+  case 6: return (1.0 - r * r / (r_max * r_max)) * (1.0 - r * r / (r_max * r_max));
+  default: return 1.0;
+  }
+}
+
+} // namespace
+
+std::vector<double> element_table(const SirProblem &p)
+{
+  const int64_t L = (int64_t) (p.G.size() / 3);
+  const double *gx = p.G.data(), *gy = gx + L, *gz = gy + L;
+  double x_max, y_max, r_max;
+  aperture_extent(gx, gy, L, x_max, y_max, r_max);
+  const double ds = p.dx * p.dy;
+  std::vector<double> t((size_t) (6 * L));
+  for (int64_t e = 0; e < L; e++) {
+    const double foc = (p.foc_met == 7) ? p.focal[(size_t) e]
+                     : focus_delay(p.foc_met, p.focal.empty() ? 0.0 : p.focal[0], gx[e], gy[e],
+                                   x_max, y_max, r_max, p.cp);
+    const double steer = steer_delay(p.steer_met, p.theta, p.phi, gx[e], gy[e], x_max, y_max, p.cp);
+    const double w = p.do_apod ? apod_weight(p.apod_met, e, p.apod.data(), gx[e], gy[e], r_max,
+                                             p.apod_par)
+                               : 1.0;
+    t[(size_t) e] = gx[e];
+    t[(size_t) (L + e)] = gy[e];
+    t[(size_t) (2 * L + e)] = gz[e];
+    t[(size_t) (3 * L + e)] = foc;
+    t[(size_t) (4 * L + e)] = steer;
+    t[(size_t) (5 * L + e)] = w * p.v * ds;   // ((w v) ds), the numerator of ai
+  }
+  return t;
+}
+
+// ------------------------------------------------------------------ plan
+
+SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int target_items)
+  : out_dtype_(out_dtype)
+{
+  require(out_dtype == DREAM_HIP_F64 || out_dtype == DREAM_HIP_F32,
+          "SIR output precision must be DREAM_HIP_F64 or DREAM_HIP_F32");
+  const size_t P = problems.size();
+  const size_t esz = out_dtype == DREAM_HIP_F64 ? 8 : 4;
+  const size_t lds_cap = 64 * 1024;
+  const int cus = device_cus();
+  if (target_items <= 0) target_items = 8 * cus;
+
+  int64_t total_no = 0;
+  for (auto &p : problems) total_no += p.No;
+
+  // Host tables and per-problem shape.
+  std::vector<std::vector<double>> xs(P), ys(P), el(P);
+  std::vector<SirDev> dev(P);
+  std::vector<double> cost(P, 0.0);
+  size_t table_doubles = 0;
+  int64_t max_nt_lds = 0;
+  for (size_t i = 0; i < P; i++) {
+    const SirProblem &p = problems[i];
+    const int64_t L = (int64_t) (p.G.size() / 3);
+    require(p.No >= 0 && p.nt >= 1, "SIR: need No >= 0 and nt >= 1");
+    require(L >= 1, "SIR: need at least one element");
+    require(p.dx > 0 && p.dy > 0 && p.dt > 0, "SIR: dx, dy and dt must be positive");
+    require(p.No == 0 || (p.Ro && p.delay && p.h), "SIR: null Ro, delay or h");
+    xs[i] = subgrid(p.a, p.dx);
+    ys[i] = subgrid(p.b, p.dy);
+    el[i] = element_table(p);
+    const int64_t nx = (int64_t) xs[i].size(), ny = (int64_t) ys[i].size();
+    require(L * ny < (1ll << 31) && nx < (1ll << 31), "SIR: element x row count too large");
+    SirDev &d = dev[i];
+    std::memset(&d, 0, sizeof d);
+    d.Ro = p.Ro;
+    d.No = p.No;
+    d.delay = p.delay;
+    d.delay_multi = p.delay_type != 0;
+    d.L = (int32_t) L;
+    d.nx = (int32_t) nx;
+    d.ny = (int32_t) ny;
+    int stride = 1;
+    if (ny > 64) {
+      stride = (int) (ny / 64);
+      while (gcd_int(stride, (int) ny) != 1) stride++;
+    }
+    d.row_stride = stride;
+    d.nt = p.nt;
+    d.cp = p.cp;
+    d.dt = p.dt;
+    d.inv_dt = 1.0 / p.dt;
+    d.c1 = 1000.0 / (p.cp * p.dt);
+    d.amp = 1000.0 / (2.0 * M_PI * p.dt);
+    d.h = p.h;
+    const int64_t Q = L * ny;
+    int64_t chunks = 1;
+    if (total_no < target_items) {
+      chunks = (target_items + std::max<int64_t>(total_no, 1) - 1) / std::max<int64_t>(total_no, 1);
+      chunks = std::min<int64_t>(chunks, std::max<int64_t>(Q / kThreads, 1));
+    }
+    d.chunks = (int32_t) std::max<int64_t>(chunks, 1);
+    const bool fits = (size_t) p.nt * esz <= lds_cap;
+    d.mode = !fits ? SIR_GLOBAL : (d.chunks > 1 ? SIR_LDS_ATOMIC : SIR_LDS_STORE);
+    if (fits) max_nt_lds = std::max(max_nt_lds, p.nt);
+    cost[i] = (double) Q * (double) nx / d.chunks;
+    table_doubles += xs[i].size() + ys[i].size() + el[i].size();
+    levels_.push_back(p.err_level);
+  }
+
+  // Largest per-item cost first: one work-group per item is then served in
+  // longest-processing-time order by the dispatcher (or by the counter).
+  std::vector<size_t> order(P);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+
+  DREAM_HIP_CHECK(hipMalloc(&d_tables_, std::max<size_t>(table_doubles, 1) * sizeof(double)));
+  DREAM_HIP_CHECK(hipMalloc(&d_status_, std::max<size_t>(P, 1) * sizeof(Status)));
+  DREAM_HIP_CHECK(hipMalloc(&d_probs_, std::max<size_t>(P, 1) * sizeof(SirDev)));
+  DREAM_HIP_CHECK(hipMalloc(&d_counter_, 2 * sizeof(unsigned long long)));
+  DREAM_HIP_CHECK(hipMemset(d_counter_, 0, 2 * sizeof(unsigned long long)));
+
+  std::vector<double> tables;
+  tables.reserve(table_doubles);
+  host_.resize(P);
+  zero_len_.assign(P, 0);
+  zero_ptr_.assign(P, nullptr);
+  int64_t item = 0;
+  for (size_t k = 0; k < P; k++) {
+    const size_t i = order[k];
+    SirDev d = dev[i];
+    d.xs = d_tables_ + tables.size();
+    tables.insert(tables.end(), xs[i].begin(), xs[i].end());
+    d.ys = d_tables_ + tables.size();
+    tables.insert(tables.end(), ys[i].begin(), ys[i].end());
+    d.el = d_tables_ + tables.size();
+    tables.insert(tables.end(), el[i].begin(), el[i].end());
+    d.status = d_status_ + i;
+    d.item_begin = item;
+    // A problem with no sub-elements (or no points) contributes no items.
+    const int64_t items = (d.nx > 0 && d.ny > 0) ? d.No * d.chunks : 0;
+    item += items;
+    host_[k] = d;
+    if (d.mode != SIR_LDS_STORE || items == 0) {
+      zero_len_[i] = d.No * d.nt;
+      zero_ptr_[i] = d.h;
+    }
+  }
+  n_items_ = item;
+  lds_bytes_ = (size_t) max_nt_lds * esz;
+  if (!tables.empty())
+    DREAM_HIP_CHECK(hipMemcpy(d_tables_, tables.data(), tables.size() * sizeof(double),
+                              hipMemcpyHostToDevice));
+  DREAM_HIP_CHECK(hipMemcpy(d_probs_, host_.data(), P * sizeof(SirDev), hipMemcpyHostToDevice));
+
+  // Persistent grid when the list is long and uneven; otherwise one
+  // work-group per item.
+  const int wg_per_cu = lds_bytes_ ? (int) std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / std::max<size_t>(lds_bytes_, 1))) : 8;
+  persistent_ = P > 1 && n_items_ > (int64_t) cus * wg_per_cu * 4;
+  grid_ = persistent_ ? cus * wg_per_cu : (int) std::min<int64_t>(n_items_, 1ll << 31);
+}
+
+SirPlan::~SirPlan()
+{
+  if (graph_) (void) hipGraphExecDestroy(graph_);
+  (void) hipFree(d_tables_);
+  (void) hipFree(d_status_);
+  (void) hipFree(d_probs_);
+  (void) hipFree(d_counter_);
+}
+
+void SirPlan::launch(hipStream_t s)
+{
+  launch_fill_zero(d_status_, host_.size() * sizeof(Status), s);
+  for (size_t i = 0; i < zero_len_.size(); i++)
+    if (zero_len_[i]) {
+      const size_t esz = out_dtype_ == DREAM_HIP_F64 ? 8 : 4;
+      launch_fill_zero(zero_ptr_[i], (size_t) zero_len_[i] * esz, s);
+    }
+  if (n_items_ == 0) return;
+  unsigned long long *ctr = persistent_ ? d_counter_ : nullptr;
+  if (out_dtype_ == DREAM_HIP_F64) {
+    hipLaunchKernelGGL(sir_kernel<double>, dim3(grid_), dim3(kThreads), lds_bytes_, s,
+                       d_probs_, (int) host_.size(), n_items_, ctr);
+  } else {
+    hipLaunchKernelGGL(sir_kernel<float>, dim3(grid_), dim3(kThreads), lds_bytes_, s,
+                       d_probs_, (int) host_.size(), n_items_, ctr);
+  }
+  DREAM_HIP_CHECK(hipGetLastError());
+}
+
+void SirPlan::run(hipStream_t s, bool use_graph)
+{
+  if (cancelled()) throw Error(DREAM_HIP_ECANCELLED, "SIR cancelled");
+  last_stream_ = s;
+  if (!use_graph) {
+    launch(s);
+    return;
+  }
+  if (!graph_) {
+    hipGraph_t g = nullptr;
+    DREAM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    launch(s);
+    DREAM_HIP_CHECK(hipStreamEndCapture(s, &g));
+    DREAM_HIP_CHECK(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+    DREAM_HIP_CHECK(hipGraphDestroy(g));
+  }
+  DREAM_HIP_CHECK(hipGraphLaunch(graph_, s));
+}
+
+void SirPlan::status(int32_t *codes, int64_t *oob)
+{
+  const size_t P = host_.size();
+  std::vector<Status> st(P);
+  if (last_stream_) DREAM_HIP_CHECK(hipStreamSynchronize(last_stream_));
+  if (P) DREAM_HIP_CHECK(hipMemcpy(st.data(), d_status_, P * sizeof(Status), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < P; i++) {
+    const int lv = levels_[i];
+    int code = E_NONE;
+    if (lv == L_STOP || lv == L_PSTOP) {
+      if (st[i].any_oob) code = E_OOB;
+    } else if (st[i].last_elem_oob) {
+      code = level_code(lv);
+    }
+    if (codes) codes[i] = code;
+    if (oob) oob[i] = (int64_t) st[i].oob;
+  }
+}
+
+void rect_indices(double xo, double yo, double zo, double a, double b, double dx, double dy,
+                  double dt, double delay, double cp, int64_t *it_out, int64_t cap,
+                  int64_t *count)
+{
+  std::vector<double> xs = subgrid(a, dx), ys = subgrid(b, dy);
+  const int64_t n = (int64_t) xs.size() * (int64_t) ys.size();
+  *count = n;
+  const int64_t m = std::min(n, cap);
+  if (m <= 0) return;
+  double *d_xy = nullptr;
+  int64_t *d_it = nullptr;
+  DREAM_HIP_CHECK(hipMalloc(&d_xy, (xs.size() + ys.size()) * sizeof(double)));
+  DREAM_HIP_CHECK(hipMalloc(&d_it, (size_t) m * sizeof(int64_t)));
+  DREAM_HIP_CHECK(hipMemcpy(d_xy, xs.data(), xs.size() * sizeof(double), hipMemcpyHostToDevice));
+  DREAM_HIP_CHECK(hipMemcpy(d_xy + xs.size(), ys.data(), ys.size() * sizeof(double),
+                            hipMemcpyHostToDevice));
+  const int blocks = (int) ((m + 255) / 256);
+  hipLaunchKernelGGL(rect_indices_kernel, dim3(blocks), dim3(256), 0, lib_stream(), xo, yo, zo,
+                     d_xy, (int) xs.size(), d_xy + xs.size(), (int) ys.size(), cp, dt, delay,
+                     d_it, m);
+  DREAM_HIP_CHECK(hipGetLastError());
+  DREAM_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  DREAM_HIP_CHECK(hipMemcpy(it_out, d_it, (size_t) m * sizeof(int64_t), hipMemcpyDeviceToHost));
+  (void) hipFree(d_xy);
+  (void) hipFree(d_it);
+}
+
+} // namespace dream

@@ -1,0 +1,88 @@
+// sir.hpp -- host side of the SIR (dreamrect / dream_arr_rect) kernels.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+
+namespace dream {
+
+// One dream_arr_rect problem as the host sees it (host copies of the
+// per-element inputs; device pointers for Ro, delay and the output).
+struct SirProblem {
+  const double *Ro = nullptr;  int64_t No = 0;          // device
+  double a = 0, b = 0, dx = 0, dy = 0, dt = 0;  int64_t nt = 0;
+  int delay_type = 0;  const double *delay = nullptr;   // device
+  double v = 1, cp = 1000;
+  std::vector<double> G;                               // L x 3 column-major (host)
+  int foc_met = 0;  std::vector<double> focal;         // 1 or L values (host)
+  int steer_met = 0;  double theta = 0, phi = 0;
+  int do_apod = 0, apod_met = 0;  double apod_par = 0;
+  std::vector<double> apod;                            // L values (host)
+  int err_level = L_STOP;
+  void *h = nullptr;                                   // device, nt x No
+};
+
+// Device descriptor of one problem (lives in device memory).
+struct SirDev {
+  const double *Ro;  int64_t No;
+  const double *delay;  int32_t delay_multi;
+  int32_t L;
+  const double *xs;  const double *ys;  int32_t nx, ny, row_stride, chunks;
+  const double *el;                 // 6 x L: gx gy gz foc steer num
+  int64_t nt;
+  double cp, dt, inv_dt, c1, amp;   // c1 = 1000/(cp dt), amp = 1000/(2 pi dt)
+  void *h;
+  Status *status;
+  int64_t item_begin;               // first work item of this problem
+  int32_t mode;                     // SIR_* below
+  int32_t pad;
+};
+
+enum { SIR_LDS_STORE = 0, SIR_LDS_ATOMIC = 1, SIR_GLOBAL = 2 };
+
+// Sub-element centres of one side: s = -w/2 + d/2; while (s <= w/2) s += d
+// (the accumulated sequence of src/dreamrect.cc:286-331).
+std::vector<double> subgrid(double w, double d);
+
+// Per-element (gx gy gz foc steer num) table of one problem, evaluated on
+// the host in the order of src/dream_arr_rect.cc:207-234.
+std::vector<double> element_table(const SirProblem &p);
+
+// A prepared batch: device tables, work list, status blocks, launch shape.
+class SirPlan {
+public:
+  SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int target_items = 0);
+  ~SirPlan();
+  void run(hipStream_t s, bool use_graph);
+  // Per-problem SIRError code and OOB counts (synchronises on the last run's stream).
+  void status(int32_t *codes, int64_t *oob);
+  int64_t n_items() const { return n_items_; }
+
+private:
+  void launch(hipStream_t s);
+  int out_dtype_;
+  std::vector<SirDev> host_;
+  std::vector<int> levels_;
+  std::vector<int64_t> zero_len_;  // per problem: elements to zero first (0 = none)
+  std::vector<void *> zero_ptr_;
+  SirDev *d_probs_ = nullptr;
+  double *d_tables_ = nullptr;
+  Status *d_status_ = nullptr;
+  unsigned long long *d_counter_ = nullptr;
+  int64_t n_items_ = 0;
+  size_t lds_bytes_ = 0;
+  int grid_ = 0;
+  bool persistent_ = false;
+  hipGraphExec_t graph_ = nullptr;
+  hipStream_t last_stream_ = nullptr;
+};
+
+// Device parity helper: per-contribution sample indices of one
+// (point, element) pair from the same device index routine.
+void rect_indices(double xo, double yo, double zo, double a, double b, double dx, double dy,
+                  double dt, double delay, double cp, int64_t *it_out, int64_t cap,
+                  int64_t *count);
+
+} // namespace dream

@@ -1,0 +1,273 @@
+"""Device-tensor API over libdream_hip.so (PyTorch-ROCm tensors as device memory).
+
+Layouts (row-major torch views of the reference's column-major matrices, so
+the bytes are the same):
+  Ro  (3, No)        rows x, y, z of the observation points / pixels
+  G   (3, L)         element positions (Gt, Gr likewise)
+  Y   (n_ascans, K) or (F, n_ascans, K)  -- A-scan a at Y[a, :]
+  Im  (No,) or (F, No)
+  H   (No, nt)       SIR of point n at H[n, :]
+
+Precisions: "f64" (FP64 data and delays, the reference's double path), "f32"
+(FP32 data and sums, FP64 geometry, delays and sample indices), "f32_ref"
+(the reference's float path, FP32 geometry).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+PREC = {"f64": _lib.F64, "f32_ref": _lib.F32_REF, "f32": _lib.F32}
+INTERP = {"nearest": _lib.NEAREST, "linear": _lib.LINEAR}
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _dev_tensor(x, dtype, device):
+    t = torch.as_tensor(x, dtype=dtype, device=device)
+    return t.contiguous()
+
+
+class DasPlan:
+    """A DAS object on the GPU (DAS<T> with cached state) for fixed geometry.
+
+    Gt, Gr, Ro: (L, 3) / (No, 3) arrays or tensors in the reference's
+    row-per-element convention; stored on the device coordinate-major.
+    """
+
+    def __init__(self, method, K, Ro, Gt, Gr=None, precision="f32", device="cuda"):
+        _lib.require_device()
+        self.method = method
+        self.type = _lib.DAS_TYPE[method]
+        self.precision = precision
+        self.prec = PREC[precision]
+        self.data_dtype = torch.float64 if precision == "f64" else torch.float32
+        self.geom_dtype = torch.float32 if precision == "f32_ref" else torch.float64
+        self.device = torch.device(device)
+        gd = self.geom_dtype
+        self.Ro = _dev_tensor(torch.as_tensor(Ro, dtype=gd).reshape(-1, 3).t(), gd, self.device)
+        self.Gt = _dev_tensor(torch.as_tensor(Gt, dtype=gd).reshape(-1, 3).t(), gd, self.device)
+        self.Lt = self.Gt.shape[1]
+        if self.type != 0:
+            self.Gr = _dev_tensor(torch.as_tensor(Gr, dtype=gd).reshape(-1, 3).t(), gd,
+                                  self.device)
+            self.Lr = self.Gr.shape[1]
+        else:
+            self.Gr = None
+            self.Lr = 0
+        self.No = self.Ro.shape[1]
+        self.K = int(K)
+        self.n_ascans = self.Lt if self.type == 0 else self.Lt * self.Lr
+        self.h = ctypes.c_void_p(None)
+        _lib.check(_lib.lib().dream_hip_das_create(self.type, self.K, self.No, self.Lt, self.Lr,
+                                                   self.prec, ctypes.byref(self.h)))
+        self._delay_cache = {}
+
+    def __del__(self):
+        try:
+            if self.h.value is not None and _lib._lib is not None:
+                _lib._lib.dream_hip_das_destroy(self.h)
+        except Exception:
+            pass
+
+    def _delay(self, delay):
+        if torch.is_tensor(delay) and delay.numel() > 1:
+            d = delay.to(device=self.device, dtype=self.geom_dtype).contiguous()
+            return d, 1
+        val = float(delay)
+        if val not in self._delay_cache:
+            self._delay_cache[val] = torch.tensor([val], dtype=self.geom_dtype, device=self.device)
+        return self._delay_cache[val], 0
+
+    def _frames(self, Y):
+        if Y.dim() == 2:
+            return 1
+        assert Y.dim() == 3, "Y must be (n_ascans, K) or (F, n_ascans, K)"
+        return Y.shape[0]
+
+    def forward(self, Y, dt, delay, cp, apod_t=None, apod_r=None, interp="nearest",
+                err_level="ignore", want_status=False, out=None):
+        """Beamform; returns Im (and the SIRError code when want_status)."""
+        assert Y.is_contiguous() and Y.dtype == self.data_dtype and Y.shape[-2:] == (
+            self.n_ascans, self.K), "Y must be contiguous (.., n_ascans, K) of the plan's dtype"
+        F = self._frames(Y)
+        d, dtype = self._delay(delay)
+        Im = out if out is not None else torch.empty(
+            (F, self.No) if Y.dim() == 3 else (self.No,), dtype=self.data_dtype, device=Y.device)
+        code = ctypes.c_int(0)
+        _lib.check(_lib.lib().dream_hip_das_dev(
+            self.h, _ptr(Y), _ptr(self.Ro), _ptr(self.Gt), _ptr(self.Gr), float(dt), dtype,
+            _ptr(d), float(cp), _ptr(apod_t), _ptr(apod_r), INTERP[interp], F,
+            self.n_ascans * self.K, self.No, _ptr(Im), _lib.ERR_LEVEL[err_level],
+            ctypes.byref(code) if want_status else None, _stream()))
+        return (Im, code.value) if want_status else Im
+
+    def adjoint(self, Y, g, dt, delay, cp, apod_t=None, apod_r=None, interp="linear",
+                want_gy=True, want_apod=True, want_cp=True):
+        """Aᵀg: returns (gY or None, g_apod_t, g_apod_r, g_cp)."""
+        F = self._frames(Y)
+        g = g.contiguous()
+        dt_ = self.data_dtype
+        gY = torch.empty_like(Y) if want_gy else None
+        gat = torch.empty(self.Lt, dtype=dt_, device=Y.device) if want_apod else None
+        gar = (torch.empty(self.Lr, dtype=dt_, device=Y.device)
+               if want_apod and self.type != 0 else None)
+        gcp = torch.empty(1, dtype=dt_, device=Y.device) if want_cp else None
+        _lib.check(_lib.lib().dream_hip_das_adjoint_dev(
+            self.h, _ptr(Y), _ptr(self.Ro), _ptr(self.Gt), _ptr(self.Gr), float(dt), float(delay),
+            float(cp), _ptr(apod_t), _ptr(apod_r), INTERP[interp], F, self.n_ascans * self.K,
+            self.No, _ptr(g), _ptr(gY), _ptr(gat), _ptr(gar), _ptr(gcp), _stream()))
+        return gY, gat, gar, gcp
+
+
+class DASFunction(torch.autograd.Function):
+    """Differentiable DAS: Im = DAS(Y; apod_t, apod_r, cp).
+
+    Gradients with respect to Y, the apodization weights and the scalar cp
+    come from the hand-written adjoint kernels.  cp is a 0-d / 1-element
+    tensor (its value is read on the host for the forward launch).
+    """
+
+    @staticmethod
+    def forward(ctx, Y, apod_t, apod_r, cp, plan, dt, delay, interp):
+        cpv = float(cp)
+        Im = plan.forward(Y.contiguous(), dt, delay, cpv, apod_t, apod_r, interp)
+        ctx.save_for_backward(Y, apod_t, apod_r)
+        ctx.plan, ctx.dt, ctx.delay, ctx.interp, ctx.cp = plan, dt, delay, interp, cpv
+        ctx.cp_shape = cp.shape if torch.is_tensor(cp) else None
+        return Im
+
+    @staticmethod
+    def backward(ctx, gIm):
+        Y, apod_t, apod_r = ctx.saved_tensors
+        want_gy = ctx.needs_input_grad[0]
+        want_apod = ctx.needs_input_grad[1] or ctx.needs_input_grad[2]
+        want_cp = ctx.needs_input_grad[3]
+        gY, gat, gar, gcp = ctx.plan.adjoint(Y.contiguous(), gIm, ctx.dt, ctx.delay, ctx.cp,
+                                             apod_t, apod_r, ctx.interp, want_gy, want_apod,
+                                             want_cp)
+        gcp_out = None
+        if want_cp and ctx.cp_shape is not None:
+            gcp_out = gcp.reshape(ctx.cp_shape)
+        return (gY, gat if ctx.needs_input_grad[1] else None,
+                gar if ctx.needs_input_grad[2] else None, gcp_out, None, None, None, None)
+
+
+def das_diff(plan, Y, apod_t, apod_r, cp, dt, delay, interp="linear"):
+    """Differentiable DAS through DASFunction."""
+    return DASFunction.apply(Y, apod_t, apod_r, cp, plan, dt, delay, interp)
+
+
+# ----------------------------------------------------------------------------- SIR
+
+def _host_f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def arr_rect(Ro, a, b, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=0.0,
+             steer_met="off", theta=0.0, phi=0.0, apod_met="off", apod=None, apod_par=0.0,
+             err_level="ignore", out_dtype=torch.float64, out=None):
+    """dream_arr_rect on device tensors.  Ro: (3, No) float64 device tensor,
+    delay: float or (No,) device tensor.  Returns (H (No, nt), SIRError code,
+    out-of-bounds contribution count)."""
+    _lib.require_device()
+    assert Ro.dtype == torch.float64 and Ro.is_contiguous() and Ro.shape[0] == 3
+    No = Ro.shape[1]
+    G = np.asarray(G, dtype=np.float64).reshape(-1, 3)
+    L = G.shape[0]
+    Gf = np.asfortranarray(G)
+    if torch.is_tensor(delay) and delay.numel() > 1:
+        d, dtype = delay.to(dtype=torch.float64).contiguous(), 1
+    else:
+        d, dtype = torch.tensor([float(delay)], dtype=torch.float64, device=Ro.device), 0
+    focal_v = _host_f64(np.atleast_1d(focal))
+    apod_v = _host_f64(np.ones(L) if apod is None else apod)
+    do_apod = 0 if apod_met == "off" else 1
+    H = out if out is not None else torch.empty((No, int(nt)), dtype=out_dtype, device=Ro.device)
+    code = ctypes.c_int(0)
+    oob = ctypes.c_int64(0)
+    _lib.check(_lib.lib().dream_hip_arr_rect_dev(
+        _ptr(Ro), No, a, b, dx, dy, dt, int(nt), dtype, _ptr(d), v, cp, L, Gf.ctypes.data,
+        _lib.FOC_MET[foc_met], focal_v.ctypes.data, _lib.STEER_MET[steer_met], theta, phi,
+        apod_v.ctypes.data, do_apod, _lib.APOD_MET.get(apod_met, 0), apod_par, _ptr(H),
+        _lib.F64 if out_dtype == torch.float64 else _lib.F32, _lib.ERR_LEVEL[err_level],
+        ctypes.byref(code), ctypes.byref(oob), _stream()))
+    return H, code.value, oob.value
+
+
+class SirBatch:
+    """Many uneven dream_arr_rect problems in one persistent launch.
+
+    problems: list of dicts with keys Ro ((3, No) float64 device tensor), a, b,
+    dx, dy, dt, nt, delay (float or (No,) device tensor), v, cp, G ((L, 3)
+    host array), and optionally foc_met, focal, steer_met, theta, phi,
+    apod_met, apod, apod_par, err_level.  Outputs H[i] are (No_i, nt_i)
+    tensors of out_dtype allocated by the batch.
+    """
+
+    def __init__(self, problems, out_dtype=torch.float32, device="cuda"):
+        _lib.require_device()
+        self.out_dtype = out_dtype
+        self.n = len(problems)
+        self._keep = []
+        self.H = []
+        arr = (_lib.DreamSirProblem * max(self.n, 1))()
+        for i, p in enumerate(problems):
+            Ro = p["Ro"]
+            assert Ro.dtype == torch.float64 and Ro.is_contiguous() and Ro.shape[0] == 3
+            No = Ro.shape[1]
+            G = np.asfortranarray(np.asarray(p["G"], dtype=np.float64).reshape(-1, 3))
+            L = G.shape[0]
+            delay = p.get("delay", 0.0)
+            if torch.is_tensor(delay) and delay.numel() > 1:
+                d, dtype = delay.to(dtype=torch.float64).contiguous(), 1
+            else:
+                d, dtype = torch.tensor([float(delay)], dtype=torch.float64, device=Ro.device), 0
+            focal = _host_f64(np.atleast_1d(p.get("focal", 0.0)))
+            apod = _host_f64(p["apod"] if p.get("apod") is not None else np.ones(L))
+            H = torch.zeros((No, int(p["nt"])), dtype=out_dtype, device=Ro.device)
+            self.H.append(H)
+            self._keep += [Ro, G, d, focal, apod, H]
+            q = arr[i]
+            q.Ro, q.No = Ro.data_ptr(), No
+            q.a, q.b, q.dx, q.dy, q.dt, q.nt = p["a"], p["b"], p["dx"], p["dy"], p["dt"], int(p["nt"])
+            q.delay_type, q.delay = dtype, d.data_ptr()
+            q.v, q.cp = p.get("v", 1.0), p["cp"]
+            q.num_elements, q.G = L, G.ctypes.data
+            q.foc_met, q.focal = _lib.FOC_MET[p.get("foc_met", "off")], focal.ctypes.data
+            q.steer_met = _lib.STEER_MET[p.get("steer_met", "off")]
+            q.theta, q.phi = p.get("theta", 0.0), p.get("phi", 0.0)
+            apod_met = p.get("apod_met", "off")
+            q.apod, q.do_apod = apod.ctypes.data, 0 if apod_met == "off" else 1
+            q.apod_met, q.apod_par = _lib.APOD_MET.get(apod_met, 0), p.get("apod_par", 0.0)
+            q.err_level = _lib.ERR_LEVEL[p.get("err_level", "ignore")]
+            q.h = H.data_ptr()
+        self.plan = ctypes.c_void_p(None)
+        _lib.check(_lib.lib().dream_hip_sir_plan_create(arr, self.n,
+                                                        _lib.F64 if out_dtype == torch.float64
+                                                        else _lib.F32, ctypes.byref(self.plan)))
+
+    def run(self, use_graph=True):
+        _lib.check(_lib.lib().dream_hip_sir_plan_run(self.plan, _stream(), 1 if use_graph else 0))
+        return self.H
+
+    def status(self):
+        codes = (ctypes.c_int32 * max(self.n, 1))()
+        oob = (ctypes.c_int64 * max(self.n, 1))()
+        _lib.check(_lib.lib().dream_hip_sir_plan_status(self.plan, codes, oob))
+        return list(codes)[:self.n], list(oob)[:self.n]
+
+    def __del__(self):
+        try:
+            if self.plan.value is not None and _lib._lib is not None:
+                _lib._lib.dream_hip_sir_plan_destroy(self.plan)
+        except Exception:
+            pass

@@ -1,0 +1,230 @@
+/*
+ * dream_hip.h -- C ABI of the MI355X (gfx950) implementation of DREAM's two
+ * data-parallel hot paths:
+ *
+ *   SIR  the discrete-representation spatial impulse response of a
+ *        rectangular element (dreamrect) and of an array of rectangular
+ *        elements (dream_arr_rect);
+ *   DAS  the delay-and-sum beamformer das (SAFT, TFM, RCA column/row
+ *        transmit), plus linear interpolation, apodization and the adjoint
+ *        for model-based imaging.
+ *
+ * Conventions (the reference's, so its gateways can bind these directly):
+ *   - matrices are column-major: Ro is No x 3 ([x | y | z]), G is L x 3,
+ *     H is nt x No (point n's SIR at H[n*nt .. n*nt+nt-1]), Y is K x n_ascans,
+ *     Im is No;
+ *   - units mm, us, m/s; dt in us; delays in us;
+ *   - enum arguments use the reference's numbering (include/dream.h:42-102):
+ *       SIRError   none=0 ignore_out_of_bounds=1 warn_out_of_bounds=2 out_of_bounds=3
+ *       ErrorLevel none=0 ignore=1 warn=2 stop=3 parallel_stop=4
+ *       DelayType  single=0 multiple=1
+ *       FocusMet   none=0 x=1 negx=2 y=3 negy=4 xy=5 x_y=6 ud=7
+ *       SteerMet   none=0 x=1 y=2 xy=3
+ *       ApodMet    triangle=0 gauss=1 raised_cosine=2 hann=3 hamming=4
+ *                  simply_supported=5 clamped=6 ud=7
+ *       DASType    saft=0 tfm=1 rca_coltx=2 rca_rowtx=3
+ *   - every function returns 0 on success and a negative DREAM_HIP_E* code
+ *     on an argument or HIP runtime error; dream_hip_last_error() then holds
+ *     the message (per thread).  The SIR/DAS out-of-bounds result is the
+ *     separate SIRError written to *sir_err.
+ *   - "_dev" entry points take device pointers for the large arrays and an
+ *     optional hipStream_t (as void*; NULL = the library's stream) and do not
+ *     synchronise; the other entry points take caller-owned host memory,
+ *     copy it to the device, and return after the result is back in host
+ *     memory, as the reference's CPU drivers do.
+ *
+ * No C++ types cross this boundary.  include/dream_hip.hpp layers the
+ * reference's C++ classes (ArrRect, Rect, DAS<T>) over it.
+ */
+#ifndef DREAM_HIP_H
+#define DREAM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes. */
+#define DREAM_HIP_OK            0
+#define DREAM_HIP_EINVAL      (-1)   /* invalid argument                       */
+#define DREAM_HIP_ERUNTIME    (-2)   /* HIP runtime error (no device, fault..)  */
+#define DREAM_HIP_ENOMEM      (-3)   /* device allocation failed               */
+#define DREAM_HIP_EUNSUPPORTED (-4)  /* feature not on this path (attenuation)  */
+#define DREAM_HIP_ECANCELLED  (-5)   /* dream_hip_abort() was called           */
+
+/* Output / data precision selectors. */
+#define DREAM_HIP_F64 0              /* FP64 data, FP64 delays (reference T=double)         */
+#define DREAM_HIP_F32_REF 1          /* FP32 data and FP32 delay arithmetic (reference T=float) */
+#define DREAM_HIP_F32 2              /* FP32 data and sums, FP64 delays and sample indices    */
+
+/* DAS sampling rules. */
+#define DREAM_HIP_NEAREST 0          /* reference rule: Y[trunc(t*fs)]                     */
+#define DREAM_HIP_LINEAR  1          /* linear interpolation, zero outside [0, K)         */
+
+/* ---------------------------------------------------------------- library */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *dream_hip_last_error(void);
+
+/* Library version string. */
+const char *dream_hip_version(void);
+
+/* Number of visible HIP devices (0 when there is none); never fails. */
+int dream_hip_device_count(void);
+
+/* Cooperative cancellation: dream_hip_abort() makes running calls stop
+ * between work chunks and return DREAM_HIP_ECANCELLED; it is safe to call
+ * from a signal handler.  Mirrors ArrRect::abort / DAS<T>::abort /
+ * set_running / is_running (include/dream_arr_rect.h:53-54, include/das.h:74-76). */
+void dream_hip_abort(int signum);
+void dream_hip_set_running(void);
+int dream_hip_is_running(void);
+
+/* ---------------------------------------------------------------- SIR */
+
+/* dream_arr_rect on the GPU.  Replaces ArrRect::dream_arr_rect
+ * (include/dream_arr_rect.h:40-51, src/dream_arr_rect.cc:310-433): same
+ * arguments in the same order.  alpha must be 0 (attenuation is not on this
+ * path: DREAM_HIP_EUNSUPPORTED otherwise).  Host pointers.  h (nt x No FP64)
+ * is overwritten (zeroed, then accumulated), which equals the reference's
+ * accumulate-into-a-zeroed-buffer contract.
+ * *sir_err: for err_level stop, out_of_bounds if any contribution fell
+ * outside [0, nt); for ignore/warn, that level's code if any observation
+ * point's last element had an out-of-bounds contribution (the reference's
+ * per-point status, src/dream_arr_rect.cc:227-239), else none. */
+int dream_hip_arr_rect(double alpha, const double *Ro, int64_t No, double a, double b,
+                       double dx, double dy, double dt, int64_t nt, int delay_type,
+                       const double *delay, double v, double cp, int64_t num_elements,
+                       const double *G, int foc_met, const double *focal, int steer_met,
+                       double theta, double phi, const double *apod, int do_apod,
+                       int apod_met, double apod_par, double *h, int err_level,
+                       int *sir_err);
+
+/* dreamrect on the GPU.  Replaces Rect::dreamrect (include/dreamrect.h:39-46,
+ * src/dreamrect.cc:169-272); host pointers; same error rule as above with one
+ * element.  Also replaces Rect::cl_dreamrect (opencl/cl_dreamrect.cc:42)
+ * without its No % 64 == 0 restriction. */
+int dream_hip_rect(double alpha, const double *Ro, int64_t No, double a, double b,
+                   double dx, double dy, double dt, int64_t nt, int delay_type,
+                   const double *delay, double v, double cp, double *h, int err_level,
+                   int *sir_err);
+
+/* Device-pointer variant of dream_hip_arr_rect.  Ro (No x 3 FP64), delay
+ * (1 or No FP64) and h are DEVICE pointers; G, focal and apod are HOST
+ * pointers (per-element delays and weights are evaluated on the host with
+ * the C library's sin/cos/exp, as the reference does).  out_dtype is
+ * DREAM_HIP_F64 or DREAM_HIP_F32 (FP32 sums, FP64 indices).  Asynchronous
+ * on `stream`; *sir_err and *oob_count (may be NULL) are written once the
+ * stream reaches the end of the call, so read them after synchronising. */
+int dream_hip_arr_rect_dev(const double *Ro, int64_t No, double a, double b, double dx,
+                           double dy, double dt, int64_t nt, int delay_type,
+                           const double *delay, double v, double cp, int64_t num_elements,
+                           const double *G, int foc_met, const double *focal,
+                           int steer_met, double theta, double phi, const double *apod,
+                           int do_apod, int apod_met, double apod_par, void *h,
+                           int out_dtype, int err_level, int *sir_err, int64_t *oob_count,
+                           void *stream);
+
+/* Per-contribution sample indices of one (observation point, element) pair,
+ * computed by the same device code as the SIR kernels: it_out[j] =
+ * rint((t_j - delay)/dt) for the j-th sub-element in the reference's (ys
+ * outer, xs inner) order.  xo,yo,zo are relative to the element centre.
+ * Host pointers; returns the contribution count in *count (writes at most
+ * cap).  For parity tests of the index path. */
+int dream_hip_rect_indices(double xo, double yo, double zo, double a, double b, double dx,
+                           double dy, double dt, double delay, double cp, int64_t *it_out,
+                           int64_t cap, int64_t *count);
+
+/* Batched uneven SIR: many dream_arr_rect problems in one persistent launch.
+ * Ro, delay and h are DEVICE pointers; G, focal, apod are HOST pointers. */
+typedef struct DreamSirProblem {
+  const double *Ro;  int64_t No;
+  double a, b, dx, dy, dt;  int64_t nt;
+  int32_t delay_type;  const double *delay;
+  double v, cp;
+  int64_t num_elements;  const double *G;
+  int32_t foc_met;  const double *focal;
+  int32_t steer_met;  double theta, phi;
+  const double *apod;  int32_t do_apod;  int32_t apod_met;  double apod_par;
+  int32_t err_level;
+  void *h;                                   /* nt x No, out_dtype */
+} DreamSirProblem;
+
+/* Build a plan (host preparation, device tables, LPT-ordered work list) for
+ * n problems; the plan owns copies of every host-side table it needs. */
+int dream_hip_sir_plan_create(const DreamSirProblem *problems, int32_t n, int out_dtype,
+                              void **plan);
+/* Launch the plan on `stream` (captured into a hipGraph on first use when
+ * use_graph != 0, replayed afterwards).  Asynchronous. */
+int dream_hip_sir_plan_run(void *plan, void *stream, int use_graph);
+/* After the stream has passed the run: per-problem SIRError codes and
+ * out-of-bounds contribution counts (either pointer may be NULL). */
+int dream_hip_sir_plan_status(void *plan, int32_t *sir_err, int64_t *oob_count);
+int dream_hip_sir_plan_destroy(void *plan);
+
+/* ---------------------------------------------------------------- DAS */
+
+/* Create a DAS object.  Replaces DAS<T>::DAS (include/das.h:57-63); the
+ * object caches device buffers and is reused while das_setup_has_changed
+ * would be false (include/das.h:77-94).  precision: DREAM_HIP_F64 (T=double),
+ * DREAM_HIP_F32_REF (T=float, the reference's float arithmetic) or
+ * DREAM_HIP_F32 (float data, double delays).  num_r_elements == 0 forces
+ * SAFT, as DAS<T>::das does (extra_src/das.cc:209-213). */
+int dream_hip_das_create(int das_type, int64_t a_scan_len, int64_t No,
+                         int64_t num_t_elements, int64_t num_r_elements, int precision,
+                         void **handle);
+int dream_hip_das_destroy(void *handle);
+
+/* 1 if (das_type, a_scan_len, No, num_t, num_r, precision) differ from the
+ * handle's (DAS<T>::das_setup_has_changed, include/das.h:77-94). */
+int dream_hip_das_setup_has_changed(void *handle, int das_type, int64_t a_scan_len,
+                                    int64_t No, int64_t num_t_elements,
+                                    int64_t num_r_elements, int precision);
+
+/* DAS<T>::das (include/das.h:67-72, extra_src/das.cc:199-298) and
+ * DAS<T>::cl_das (opencl/cl_das.cc:62-159) on the GPU.  HOST pointers of the
+ * handle's element type T (double for F64, float otherwise): Y (K x n_ascans),
+ * Ro (No x 3), Gt (Lt x 3), Gr (Lr x 3; ignored for SAFT), delay (1 or No).
+ * dt, delay and cp are T values passed as double.  Im (No) is overwritten.
+ * *sir_err: the err_level code if any pixel-channel sample index fell
+ * outside [0, K), else none (stop -> out_of_bounds). */
+int dream_hip_das(void *handle, const void *Y, const void *Ro, const void *Gt,
+                  const void *Gr, double dt, int delay_type, const void *delay, double cp,
+                  void *Im, int err_level, int *sir_err);
+
+/* Device-pointer DAS with the extensions.  Y, Ro, Gt, Gr, delay, apod_t,
+ * apod_r and Im are DEVICE pointers of the handle's element type (geometry
+ * and delay in double for DREAM_HIP_F32, which keeps delays in FP64).
+ * interp: DREAM_HIP_NEAREST (reference) or DREAM_HIP_LINEAR.  apod_t
+ * (Lt) / apod_r (Lr, unused by SAFT) may be NULL for all-ones; SAFT weights
+ * A-scan n by apod_t[n], TFM/RCA weight pair (i,j) by apod_t[i]*apod_r[j].
+ * n_frames >= 1 frames are beamformed in one launch: frame f reads
+ * Y + f*y_frame_stride and writes Im + f*im_frame_stride (element strides).
+ * Asynchronous; *sir_err (may be NULL) is written when the stream passes the
+ * end of the call. */
+int dream_hip_das_dev(void *handle, const void *Y, const void *Ro, const void *Gt,
+                      const void *Gr, double dt, int delay_type, const void *delay,
+                      double cp, const void *apod_t, const void *apod_r, int interp,
+                      int64_t n_frames, int64_t y_frame_stride, int64_t im_frame_stride,
+                      void *Im, int err_level, int *sir_err, void *stream);
+
+/* Adjoint of dream_hip_das_dev (linear or nearest rule) for the cotangent
+ * g (No per frame, DEVICE).  Outputs (DEVICE, NULL to skip; all overwritten):
+ *   gY       K x n_ascans per frame (same layout and strides as Y),
+ *   g_apod_t Lt, g_apod_r Lr (summed over frames),
+ *   g_cp     one value of the handle's element type (summed over frames;
+ *            0 for the nearest rule, whose index is piecewise constant).
+ * Not available for DREAM_HIP_F32_REF handles. */
+int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const void *Gt,
+                              const void *Gr, double dt, double delay, double cp,
+                              const void *apod_t, const void *apod_r, int interp,
+                              int64_t n_frames, int64_t y_frame_stride,
+                              int64_t im_frame_stride, const void *g, void *gY,
+                              void *g_apod_t, void *g_apod_r, void *g_cp, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DREAM_HIP_H */

@@ -1,0 +1,283 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+import this module, and only as the checker or the timed CPU baseline; the
+product package ``dream_amd`` never imports it.
+
+Two libraries are wrapped:
+
+* ``oracle/libdream_oracle.so`` -- this project's C++ restatement of the
+  reference's CPU paths (``oracle/dream_oracle.cc``).
+* ``oracle/_ref/libdream_ref.so`` -- the reference's own ``das.cc`` and
+  ``arr_functions.cc`` compiled from source (``make -C oracle ref``), present
+  only where the reference tree was available at build time.
+
+Array conventions follow the reference: column-major (Fortran) matrices, ``Ro``
+is No x 3, ``G`` is L x 3, ``H`` is nt x No, ``Y`` is K x n_ascans.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdream_oracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libdream_ref.so")
+
+# Enum numbering of include/dream.h:42-102.
+ERR_LEVEL = {"none": 0, "ignore": 1, "warn": 2, "stop": 3, "parallel_stop": 4}
+FOC = {"off": 0, "x": 1, "y": 3, "xy": 5, "on": 5, "x+y": 6, "ud": 7}
+STEER = {"off": 0, "x": 1, "y": 2, "xy": 3}
+APOD = {"triangle": 0, "gauss": 1, "raised": 2, "hann": 3, "hamming": 4,
+        "simply": 5, "clamped": 6, "ud": 7}
+DAS_TYPE = {"saft": 0, "tfm": 1, "rca": 2, "rca_coltx": 2, "rca_rowtx": 3}
+
+_c_i64 = ctypes.c_int64
+_c_dp = ctypes.POINTER(ctypes.c_double)
+_c_fp = ctypes.POINTER(ctypes.c_float)
+_c_lp = ctypes.POINTER(ctypes.c_longlong)
+_c_i64p = ctypes.POINTER(ctypes.c_int64)
+
+_lib = None
+_ref = None
+
+
+def build():
+    """Compile the oracle (and the reference checker when the tree exists)."""
+    subprocess.run(["make", "-s", "-C", HERE, "libdream_oracle.so"], check=True)
+    if os.path.isdir(os.environ.get("DREAM_REFERENCE", "/root/reference")):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.run(["make", "-s", "-C", HERE, "libdream_oracle.so"], check=True)
+        L = ctypes.CDLL(LIB_PATH)
+        L.dro_subgrid.restype = _c_i64
+        L.dro_subgrid.argtypes = [ctypes.c_double, ctypes.c_double, _c_dp, _c_i64]
+        L.dro_max_dim_arr.argtypes = [_c_dp, _c_dp, _c_i64, _c_dp, _c_dp, _c_dp]
+        L.dro_focusing.restype = ctypes.c_double
+        L.dro_focusing.argtypes = [ctypes.c_int] + [ctypes.c_double] * 7
+        L.dro_beamsteering.restype = ctypes.c_double
+        L.dro_beamsteering.argtypes = [ctypes.c_int] + [ctypes.c_double] * 7
+        L.dro_apodization.restype = ctypes.c_double
+        L.dro_apodization.argtypes = [ctypes.c_int, _c_i64, _c_dp] + [ctypes.c_double] * 5
+        L.dro_rect_contributions.restype = _c_i64
+        L.dro_rect_contributions.argtypes = [ctypes.c_double] * 12 + [_c_i64p, _c_dp, _c_i64]
+        L.dro_dreamrect.restype = ctypes.c_int
+        L.dro_dreamrect.argtypes = [_c_dp, _c_i64] + [ctypes.c_double] * 5 + [
+            _c_i64, ctypes.c_int, _c_dp, ctypes.c_double, ctypes.c_double, _c_dp,
+            ctypes.c_int, ctypes.c_int, _c_lp]
+        L.dro_dream_arr_rect.restype = ctypes.c_int
+        L.dro_dream_arr_rect.argtypes = [_c_dp, _c_i64] + [ctypes.c_double] * 5 + [
+            _c_i64, ctypes.c_int, _c_dp, ctypes.c_double, ctypes.c_double, _c_i64, _c_dp,
+            ctypes.c_int, _c_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _c_dp,
+            ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_dp, ctypes.c_int, ctypes.c_int,
+            _c_lp]
+        for name, ptr, scal in (("dro_das_f64", _c_dp, ctypes.c_double),
+                                ("dro_das_f32", _c_fp, ctypes.c_float)):
+            f = getattr(L, name)
+            f.restype = ctypes.c_int
+            f.argtypes = [ctypes.c_int, ptr, _c_i64, ptr, _c_i64, ptr, _c_i64, ptr, _c_i64,
+                          scal, ctypes.c_int, ptr, scal, ptr, ctypes.c_int, ctypes.c_int]
+        L.dro_das_ext_f64.restype = ctypes.c_int
+        L.dro_das_ext_f64.argtypes = [ctypes.c_int, ctypes.c_int, _c_dp, _c_i64, _c_dp, _c_i64,
+                                      _c_dp, _c_i64, _c_dp, _c_i64, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, _c_dp, _c_dp, _c_dp,
+                                      ctypes.c_int]
+        L.dro_das_ext_adj_f64.restype = None
+        L.dro_das_ext_adj_f64.argtypes = [ctypes.c_int, ctypes.c_int, _c_dp, _c_i64, _c_dp,
+                                          _c_i64, _c_dp, _c_i64, _c_dp, _c_i64,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp,
+                                          ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def have_ref():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """The reference-backed checker (das.cc + arr_functions.cc from source)."""
+    global _ref
+    if _ref is None:
+        R = ctypes.CDLL(REF_PATH)
+        for name, ptr, scal in (("ref_das_f64", _c_dp, ctypes.c_double),
+                                ("ref_das_f32", _c_fp, ctypes.c_float)):
+            f = getattr(R, name)
+            f.restype = ctypes.c_int
+            f.argtypes = [ctypes.c_int, ptr, _c_i64, ptr, _c_i64, ptr, _c_i64, ptr, _c_i64,
+                          scal, ctypes.c_int, ptr, scal, ptr, ctypes.c_int]
+        R.ref_max_dim_arr.argtypes = [_c_dp, _c_dp, _c_dp, _c_i64, _c_dp, _c_dp, _c_dp]
+        R.ref_focusing.restype = ctypes.c_double
+        R.ref_focusing.argtypes = [ctypes.c_int] + [ctypes.c_double] * 7
+        R.ref_beamsteering.restype = ctypes.c_double
+        R.ref_beamsteering.argtypes = [ctypes.c_int] + [ctypes.c_double] * 8
+        R.ref_apodization.restype = ctypes.c_double
+        R.ref_apodization.argtypes = [ctypes.c_int, _c_i64, _c_dp] + [ctypes.c_double] * 5
+        _ref = R
+    return _ref
+
+
+def _f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _f32(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float32))
+
+
+def _p(a, ctype=ctypes.c_double):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+# ----------------------------------------------------------------------------- helpers
+
+def subgrid(w, d):
+    """Sub-element centres of one rectangle side (src/dreamrect.cc:286-331)."""
+    n = lib().dro_subgrid(float(w), float(d), None, 0)
+    out = np.empty(n, dtype=np.float64)
+    lib().dro_subgrid(float(w), float(d), _p(out), n)
+    return out
+
+
+def max_dim_arr(G):
+    G = _f64(G)
+    xm, ym, rm = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib().dro_max_dim_arr(_p(G[:, 0].copy()), _p(G[:, 1].copy()), G.shape[0],
+                          ctypes.byref(xm), ctypes.byref(ym), ctypes.byref(rm))
+    return xm.value, ym.value, rm.value
+
+
+def rect_contributions(xo, yo, zo, a, b, dx, dy, dt, delay, v, cp, weight=1.0):
+    """Per-contribution (sample index, amplitude) of one rectangle sum."""
+    L = lib()
+    args = [float(x) for x in (xo, yo, zo, a, b, dx, dy, dt, delay, v, cp, weight)]
+    n = L.dro_rect_contributions(*args, None, None, 0)
+    it = np.empty(n, dtype=np.int64)
+    ai = np.empty(n, dtype=np.float64)
+    L.dro_rect_contributions(*args, it.ctypes.data_as(_c_i64p), _p(ai), n)
+    return it, ai
+
+
+# ----------------------------------------------------------------------------- SIR
+
+def dreamrect(Ro, a, b, dx, dy, dt, nt, delay, v, cp, err_level="stop", nthreads=0):
+    """Rect::dreamrect (no attenuation).  Returns (H nt x No, err code, oob count)."""
+    Ro = _f64(np.atleast_2d(Ro))
+    No = Ro.shape[0]
+    delay = _f64(np.atleast_1d(np.asarray(delay, dtype=np.float64).ravel()))
+    dtype = 0 if delay.size == 1 else 1
+    H = np.zeros((int(nt), No), dtype=np.float64, order="F")
+    oob = ctypes.c_longlong(0)
+    err = lib().dro_dreamrect(_p(Ro), No, a, b, dx, dy, dt, int(nt), dtype, _p(delay), v, cp,
+                              _p(H), ERR_LEVEL[err_level], int(nthreads), ctypes.byref(oob))
+    return H, err, oob.value
+
+
+def dream_arr_rect(Ro, a, b, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=0.0,
+                   steer_met="off", theta=0.0, phi=0.0, apod_met="off", apod=None,
+                   apod_par=0.0, err_level="stop", nthreads=0):
+    """ArrRect::dream_arr_rect (no attenuation).  Returns (H, err code, oob count)."""
+    Ro = _f64(np.atleast_2d(Ro))
+    G = _f64(np.atleast_2d(G))
+    No, L = Ro.shape[0], G.shape[0]
+    delay = _f64(np.atleast_1d(np.asarray(delay, dtype=np.float64).ravel()))
+    dtype = 0 if delay.size == 1 else 1
+    focal = _f64(np.atleast_1d(np.asarray(focal, dtype=np.float64).ravel()))
+    if focal.size < L:
+        focal = _f64(np.resize(focal, L))
+    do_apod = apod_met != "off"
+    apod_v = _f64(np.ones(L) if apod is None else np.asarray(apod, dtype=np.float64).ravel())
+    H = np.zeros((int(nt), No), dtype=np.float64, order="F")
+    oob = ctypes.c_longlong(0)
+    err = lib().dro_dream_arr_rect(
+        _p(Ro), No, a, b, dx, dy, dt, int(nt), dtype, _p(delay), v, cp, L, _p(G),
+        FOC[foc_met], _p(focal), STEER[steer_met], theta, phi, _p(apod_v), int(do_apod),
+        APOD.get(apod_met, 0), apod_par, _p(H), ERR_LEVEL[err_level], int(nthreads),
+        ctypes.byref(oob))
+    return H, err, oob.value
+
+
+# ----------------------------------------------------------------------------- DAS
+
+def _das_args(Y, Gt, Gr, Ro, delay, conv):
+    Y = conv(Y)
+    Gt = conv(np.atleast_2d(Gt))
+    Gr = conv(np.atleast_2d(Gr)) if Gr is not None and np.size(Gr) else None
+    Ro = conv(np.atleast_2d(Ro))
+    delay = conv(np.atleast_1d(np.asarray(delay).ravel()))
+    return Y, Gt, Gr, Ro, delay
+
+
+def das(Y, Gt, Gr, Ro, dt, delay, cp, method="tfm", err_level="ignore", precision="f64",
+        nthreads=0, use_ref=False):
+    """DAS<T>::das.  precision 'f64' or 'f32' selects T.  Returns (Im, err code).
+
+    use_ref=True runs the reference's own das.cc (oracle/_ref) instead of the
+    restatement.
+    """
+    f32 = precision == "f32"
+    conv, ptr = (_f32, ctypes.c_float) if f32 else (_f64, ctypes.c_double)
+    Y, Gt, Gr, Ro, delay = _das_args(Y, Gt, Gr, Ro, delay, conv)
+    K = Y.shape[0]
+    Lt = Gt.shape[0]
+    Lr = 0 if Gr is None or DAS_TYPE[method] == 0 else Gr.shape[0]
+    No = Ro.shape[0]
+    dtype = 0 if delay.size == 1 else 1
+    Im = np.zeros(No, dtype=np.float32 if f32 else np.float64)
+    gr_p = _p(Gr, ptr) if Gr is not None else None
+    if use_ref:
+        fn = ref().ref_das_f32 if f32 else ref().ref_das_f64
+        err = fn(DAS_TYPE[method], _p(Y, ptr), K, _p(Ro, ptr), No, _p(Gt, ptr), Lt, gr_p, Lr,
+                 dt, dtype, _p(delay, ptr), cp, _p(Im, ptr), ERR_LEVEL[err_level])
+    else:
+        fn = lib().dro_das_f32 if f32 else lib().dro_das_f64
+        err = fn(DAS_TYPE[method], _p(Y, ptr), K, _p(Ro, ptr), No, _p(Gt, ptr), Lt, gr_p, Lr,
+                 dt, dtype, _p(delay, ptr), cp, _p(Im, ptr), ERR_LEVEL[err_level],
+                 int(nthreads))
+    return Im, err
+
+
+def das_ext(Y, Gt, Gr, Ro, dt, delay, cp, method="saft", interp="linear", apod_t=None,
+            apod_r=None, nthreads=0):
+    """Extension forward (FP64): linear interpolation + apodization.
+
+    Returns (Im, any_oob).
+    """
+    Y, Gt, Gr, Ro, _ = _das_args(Y, Gt, Gr, Ro, [0.0], _f64)
+    K, Lt, No = Y.shape[0], Gt.shape[0], Ro.shape[0]
+    Lr = 0 if Gr is None or DAS_TYPE[method] == 0 else Gr.shape[0]
+    at = None if apod_t is None else _f64(np.asarray(apod_t).ravel())
+    ar = None if apod_r is None else _f64(np.asarray(apod_r).ravel())
+    Im = np.zeros(No, dtype=np.float64)
+    oob = lib().dro_das_ext_f64(DAS_TYPE[method], 1 if interp == "linear" else 0, _p(Y), K,
+                                _p(Ro), No, _p(Gt), Lt, _p(Gr), Lr, float(dt), float(delay),
+                                float(cp), _p(at), _p(ar), _p(Im), int(nthreads))
+    return Im, oob
+
+
+def das_ext_adjoint(Y, Gt, Gr, Ro, dt, delay, cp, g, method="saft", interp="linear",
+                    apod_t=None, apod_r=None, want_gy=True, nthreads=0):
+    """Adjoint of das_ext for cotangent g.  Returns (gY, g_apod_t, g_apod_r, g_cp)."""
+    Y, Gt, Gr, Ro, _ = _das_args(Y, Gt, Gr, Ro, [0.0], _f64)
+    K, Lt, No = Y.shape[0], Gt.shape[0], Ro.shape[0]
+    Lr = 0 if Gr is None or DAS_TYPE[method] == 0 else Gr.shape[0]
+    at = None if apod_t is None else _f64(np.asarray(apod_t).ravel())
+    ar = None if apod_r is None else _f64(np.asarray(apod_r).ravel())
+    g = _f64(np.asarray(g).ravel())
+    n_ascans = Lt if Lr == 0 else Lt * Lr
+    gY = np.zeros((K, n_ascans), dtype=np.float64, order="F") if want_gy else None
+    gat = np.zeros(Lt)
+    gar = np.zeros(max(Lr, 1))
+    gcp = ctypes.c_double(0.0)
+    lib().dro_das_ext_adj_f64(DAS_TYPE[method], 1 if interp == "linear" else 0, _p(Y), K,
+                              _p(Ro), No, _p(Gt), Lt, _p(Gr), Lr, float(dt), float(delay),
+                              float(cp), _p(at), _p(ar), _p(g), _p(gY), _p(gat), _p(gar),
+                              ctypes.byref(gcp), int(nthreads))
+    return gY, gat, (gar[:Lr] if Lr else None), gcp.value

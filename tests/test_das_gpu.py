@@ -1,0 +1,199 @@
+"""GPU parity of the DAS kernels against the oracle and the reference's vectors.
+
+Bars: FP64 -- the reference's sample indices bit for bit (ramp data) and
+<= 1e-12 relative error; FP32 reference arithmetic (das_f) -- the reference's
+float indices bit for bit; FP32 data with FP64 delays -- the FP64 indices bit
+for bit and <= 2e-5 relative L2 error against the FP64 oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+METHODS = ("saft", "tfm", "rca_coltx", "rca_rowtx")
+
+
+def _golden():
+    d = np.load(os.path.join(HERE, "golden", "das_ref.npz"))
+    names = sorted({k.split("__")[0] for k in d.files if k.endswith("__delay")})
+    return d, names
+
+
+def _setup(method, K=700, L=9, No=500, seed=0):
+    rng = np.random.default_rng(seed)
+    x = np.linspace(-4.0, 4.0, L)
+    Gt = np.c_[x, np.zeros(L), np.zeros(L)]
+    Gr = Gt.copy() if method == "tfm" else np.c_[np.zeros(L), x, np.zeros(L)]
+    n_ascans = L if method == "saft" else L * L
+    Y = rng.standard_normal((K, n_ascans))
+    Ro = np.c_[rng.uniform(-6, 6, No), rng.uniform(-5, 5, No), rng.uniform(0.5, 9, No)]
+    return Y, Gt, Gr, Ro, rng
+
+
+def test_golden_reference_vectors(gpu):
+    """das / das_f on the GPU against the reference's own outputs."""
+    from dream_amd import das, das_f
+    d, names = _golden()
+    for n in names:
+        m = next(x for x in METHODS if n.startswith(x + "_"))
+        level = n.rsplit("_", 1)[1]
+        dt, cp = d[f"{n}__scalars"]
+        delay = d[f"{n}__delay"]
+        dl = delay if delay.size > 1 else float(delay[0])
+        Y, Gt, Gr, Ro = d[f"{m}__Y"], d[f"{m}__Gt"], d[f"{m}__Gr"], d[f"{m}__Ro"]
+        Im = das.das(Y, Gt, Gr, Ro, dt, dl, cp, m, level, "gpu")
+        ref = d[f"{n}__f64__Im"]
+        scale = np.abs(Y).max() * Y.shape[1]
+        assert np.max(np.abs(Im - ref)) <= 1e-13 * scale, n
+        Imf = das_f.das(Y.astype(np.float32), Gt.astype(np.float32), Gr.astype(np.float32),
+                        Ro.astype(np.float32), np.float32(dt), np.float32(dl) if np.isscalar(dl)
+                        else dl.astype(np.float32), np.float32(cp), m, level, "gpu")
+        reff = d[f"{n}__f32__Im"]
+        assert np.max(np.abs(Imf.astype(np.float64) - reff)) <= 2e-6 * scale, n
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_ramp_indices_exact_all_precisions(gpu, oracle, method):
+    """Y[a, k] = k: sums are exact integers in any order, so equality means
+    equal sample-index multisets per pixel."""
+    from dream_amd import das, das_f
+    from dream_amd.torch_ops import DasPlan
+    _, Gt, Gr, Ro, rng = _setup(method, K=900, No=3000, seed=1)
+    n_ascans = Gt.shape[0] if method == "saft" else Gt.shape[0] * Gr.shape[0]
+    ramp = np.repeat(np.arange(900, dtype=np.float64)[:, None], n_ascans, axis=1)
+    for delay in (0.0, 0.31, rng.uniform(-0.4, 0.9, Ro.shape[0])):
+        ref64, _ = oracle.das(ramp, Gt, Gr, Ro, 0.02, delay, 1540.0, method, "ignore", "f64")
+        gpu64 = das.das(ramp, Gt, Gr, Ro, 0.02, delay, 1540.0, method, "ignore", "gpu")
+        assert np.array_equal(gpu64, ref64)
+        ref32, _ = oracle.das(ramp, Gt, Gr, Ro, 0.02, delay, 1540.0, method, "ignore", "f32")
+        gpu32 = das_f.das(ramp, Gt, Gr, Ro, 0.02, delay, 1540.0, method, "ignore", "gpu")
+        assert np.array_equal(gpu32, ref32)
+        # FP32 data, FP64 delays: the FP64 indices.
+        plan = DasPlan(method, 900, Ro, Gt, Gr, precision="f32")
+        Yt = torch.tensor(ramp.T.copy(), dtype=torch.float32, device="cuda")
+        dl = torch.tensor(delay, device="cuda") if np.ndim(delay) else delay
+        Im = plan.forward(Yt, 0.02, dl, 1540.0)
+        assert np.array_equal(Im.cpu().numpy().astype(np.float64), ref64)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_random_data_f64_and_error_codes(gpu, oracle, method):
+    from dream_amd import das
+    Y, Gt, Gr, Ro, _ = _setup(method, seed=2)
+    for level, code in (("ignore", 1), ("warn", 2)):
+        ref, err = oracle.das(Y, Gt, Gr, Ro, 0.02, 0.1, 1500.0, method, level, "f64", nthreads=4)
+        Im = das.das(Y, Gt, Gr, Ro, 0.02, 0.1, 1500.0, method, level, "gpu")
+        assert err == code
+        assert np.max(np.abs(Im - ref)) <= 1e-12 * np.abs(ref).max()
+    with pytest.raises(RuntimeError):
+        das.das(Y, Gt, Gr, Ro, 0.02, 0.1, 1500.0, method, "stop", "gpu")
+    # In-bounds everywhere: no error even with "stop".
+    Ro_in = Ro.copy()
+    Ro_in[:, 2] = np.clip(Ro_in[:, 2], 0.5, 2.0)
+    Ro_in[:, :2] *= 0.1
+    Yl = np.vstack([Y] * 3)
+    ref, err = oracle.das(Yl, Gt, Gr, Ro_in, 0.02, 0.0, 1500.0, method, "stop", "f64", nthreads=4)
+    assert err == 0
+    Im = das.das(Yl, Gt, Gr, Ro_in, 0.02, 0.0, 1500.0, method, "stop", "gpu")
+    assert np.max(np.abs(Im - ref)) <= 1e-12 * np.abs(ref).max()
+
+
+def test_c2_tfm_f32_vs_f64_oracle(gpu, oracle):
+    """The C2 TFM configuration: FP32 data / FP64 delays within 2e-5 rel-L2 of
+    the FP64 oracle; FP64 within 1e-12; reference-float path against the
+    reference-float oracle."""
+    import workloads
+    from dream_amd.torch_ops import DasPlan
+    w = workloads.c2(device="cuda", dtype=torch.float64)
+    Y64 = w["Y"]
+    Y_np = Y64.cpu().numpy().T.copy()            # K x n_ascans
+    ref, _ = oracle.das(Y_np, w["Gt"], w["Gr"], w["Ro"], w["dt"], 0.0, w["cp"], "tfm", "ignore",
+                        "f64")
+    p32 = DasPlan("tfm", w["K"], w["Ro"], w["Gt"], w["Gr"], precision="f32")
+    Im32 = p32.forward(Y64.float().contiguous(), w["dt"], 0.0, w["cp"]).double().cpu().numpy()
+    # Y rounded to FP32 is part of the FP32 path; the bar includes it.
+    assert rel_l2(Im32, ref) <= 2e-5
+    p64 = DasPlan("tfm", w["K"], w["Ro"], w["Gt"], w["Gr"], precision="f64")
+    Im64 = p64.forward(Y64, w["dt"], 0.0, w["cp"]).cpu().numpy()
+    assert np.max(np.abs(Im64 - ref)) <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_linear_apod_forward_and_adjoint(gpu, oracle, method):
+    from dream_amd.torch_ops import DasPlan
+    Y, Gt, Gr, Ro, rng = _setup(method, seed=3, No=700)
+    Lr = 0 if method == "saft" else Gr.shape[0]
+    at = rng.uniform(0.5, 1.5, Gt.shape[0])
+    ar = rng.uniform(0.5, 1.5, Lr) if Lr else None
+    g = rng.standard_normal(Ro.shape[0])
+    ref, _ = oracle.das_ext(Y, Gt, Gr, Ro, 0.02, 0.1, 1500.0, method, "linear", at, ar)
+    gY, gat, gar, gcp = oracle.das_ext_adjoint(Y, Gt, Gr, Ro, 0.02, 0.1, 1500.0, g, method,
+                                               "linear", at, ar)
+    for prec, dt_, tol in (("f64", torch.float64, 1e-12), ("f32", torch.float32, 2e-5)):
+        plan = DasPlan(method, Y.shape[0], Ro, Gt, Gr, precision=prec)
+        Yt = torch.tensor(Y.T.copy(), dtype=dt_, device="cuda")
+        att = torch.tensor(at, dtype=dt_, device="cuda")
+        art = torch.tensor(ar, dtype=dt_, device="cuda") if Lr else None
+        Im = plan.forward(Yt, 0.02, 0.1, 1500.0, att, art, "linear").double().cpu().numpy()
+        assert rel_l2(Im, ref) <= tol
+        gt = torch.tensor(g, dtype=dt_, device="cuda")
+        rY, rat, rar, rcp = plan.adjoint(Yt, gt, 0.02, 0.1, 1500.0, att, art, "linear")
+        atol = 1e-10 if prec == "f64" else 5e-5
+        assert rel_l2(rY.double().cpu().numpy().T, gY) <= atol
+        assert rel_l2(rat.double().cpu().numpy(), gat) <= atol
+        if Lr:
+            assert rel_l2(rar.double().cpu().numpy(), gar) <= atol
+        assert abs(float(rcp) - gcp) <= (1e-9 if prec == "f64" else 1e-3) * abs(gcp)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_dot_product_identity_gpu(gpu, method):
+    from dream_amd.torch_ops import DasPlan
+    Y, Gt, Gr, Ro, rng = _setup(method, seed=4, No=900)
+    plan = DasPlan(method, Y.shape[0], Ro, Gt, Gr, precision="f64")
+    y = torch.tensor(rng.standard_normal(Y.T.shape), device="cuda")
+    g = torch.tensor(rng.standard_normal(Ro.shape[0]), device="cuda")
+    for interp in ("linear", "nearest"):
+        Ay = plan.forward(y, 0.02, 0.1, 1500.0, interp=interp)
+        ATg, _, _, _ = plan.adjoint(y, g, 0.02, 0.1, 1500.0, interp=interp)
+        lhs, rhs = float(Ay @ g), float((y * ATg).sum())
+        assert abs(lhs - rhs) <= 1e-12 * float(Ay.norm() * g.norm())
+
+
+def test_frames_equal_single_calls(gpu):
+    from dream_amd.torch_ops import DasPlan
+    method = "rca_rowtx"
+    Y, Gt, Gr, Ro, rng = _setup(method, seed=5, No=1000)
+    F = 3
+    Yf = torch.tensor(np.stack([Y.T * (f + 1) for f in range(F)]), dtype=torch.float32,
+                      device="cuda").contiguous()
+    plan = DasPlan(method, Y.shape[0], Ro, Gt, Gr, precision="f32")
+    Im = plan.forward(Yf, 0.02, 0.1, 1500.0)
+    for f in range(F):
+        single = plan.forward(Yf[f].contiguous(), 0.02, 0.1, 1500.0)
+        assert torch.equal(Im[f], single)
+
+
+def test_autograd_function(gpu, oracle):
+    from dream_amd.torch_ops import DasPlan, das_diff
+    method = "saft"
+    Y, Gt, Gr, Ro, rng = _setup(method, seed=6, No=800)
+    plan = DasPlan(method, Y.shape[0], Ro, Gt, Gr, precision="f64")
+    Yt = torch.tensor(Y.T.copy(), device="cuda", requires_grad=True)
+    at = torch.tensor(rng.uniform(0.5, 1.5, Gt.shape[0]), device="cuda", requires_grad=True)
+    cp = torch.tensor(1500.0, dtype=torch.float64, requires_grad=True)
+    target = torch.tensor(rng.standard_normal(Ro.shape[0]), device="cuda")
+    Im = das_diff(plan, Yt, at, None, cp, 0.02, 0.1, "linear")
+    loss = ((Im - target) ** 2).sum()
+    loss.backward()
+    g = 2 * (Im - target).detach().cpu().numpy()
+    gY, gat, _, gcp = oracle.das_ext_adjoint(Y, Gt, None, Ro, 0.02, 0.1, 1500.0, g, method,
+                                             "linear", at.detach().cpu().numpy())
+    assert rel_l2(Yt.grad.cpu().numpy().T, gY) <= 1e-10
+    assert rel_l2(at.grad.cpu().numpy(), gat) <= 1e-10
+    assert abs(float(cp.grad) - gcp) <= 1e-9 * abs(gcp)

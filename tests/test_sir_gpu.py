@@ -1,0 +1,206 @@
+"""GPU parity of the SIR kernels (dreamrect / dream_arr_rect) against the oracle.
+
+Bars: per-contribution sample indices bit for bit (direct dump, including
+contributions placed on rounding boundaries), FP64 output within 1e-12 of
+the oracle (an index moved by one sample would change a bin by ~1e-3), FP32
+output within 2e-5 relative L2, the reference's error codes.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_indices(xo, yo, zo, a, b, dx, dy, dt, delay, cp):
+    from dream_amd import _lib
+    L = _lib.lib()
+    cnt = ctypes.c_int64(0)
+    _lib.check(L.dream_hip_rect_indices(xo, yo, zo, a, b, dx, dy, dt, delay, cp, None, 0,
+                                        ctypes.byref(cnt)))
+    out = np.empty(cnt.value, dtype=np.int64)
+    _lib.check(L.dream_hip_rect_indices(xo, yo, zo, a, b, dx, dy, dt, delay, cp,
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                        cnt.value, ctypes.byref(cnt)))
+    return out
+
+
+def test_contribution_indices_bit_exact(gpu, oracle):
+    rng = np.random.default_rng(10)
+    n_boundary = 0
+    for trial in range(60):
+        a, b = rng.uniform(0.2, 1.5), rng.uniform(2.0, 15.0)
+        dx = dy = rng.choice([0.05, 0.03, 0.1])
+        dt = rng.choice([0.02, 0.025, 0.01])
+        cp = rng.uniform(1400, 1600)
+        xo, yo, zo = rng.uniform(-8, 8), rng.uniform(-8, 8), rng.uniform(1, 80)
+        delay = rng.uniform(-5, 20)
+        if trial % 2:
+            # Put one contribution on the rounding boundary: (t - delay)/dt = k + 1/2.
+            it, _ = oracle.rect_contributions(xo, yo, zo, a, b, dx, dy, dt, 0.0, 1.0, cp)
+            t0 = float(it[len(it) // 2]) * dt
+            xs, ys = oracle.subgrid(a, dx), oracle.subgrid(b, dy)
+            j = len(it) // 2
+            rx, ry = xo - xs[j % len(xs)], yo - ys[j // len(xs)]
+            t = np.sqrt(rx * rx + ry * ry + zo * zo) * 1000.0 / cp
+            delay = t - (np.floor(t / dt) + 0.5) * dt
+            n_boundary += 1
+        ref, _ = oracle.rect_contributions(xo, yo, zo, a, b, dx, dy, dt, delay, 1.0, cp)
+        got = _gpu_indices(xo, yo, zo, a, b, dx, dy, dt, delay, cp)
+        assert np.array_equal(got, ref), (trial, np.nonzero(got != ref)[0][:5])
+    assert n_boundary >= 20
+
+
+def _gpu_arr_rect(w, **kw):
+    from dream_amd import dream_arr_rect
+    args = dict(foc_met="off", focal=0.0, steer_met="off", steer_par=[0.0, 0.0], apod_met="off",
+                apod=None, win_par=0.0)
+    args.update(kw)
+    return dream_arr_rect.dream_arr_rect(
+        w["Ro"], [w["a"], w["b"]], w["G"], [w["dx"], w["dy"], w["dt"], w["nt"]], w["delay"],
+        [w["v"], w["cp"], 0.0], args["foc_met"], args["focal"], args["steer_met"],
+        args["steer_par"], args["apod_met"], args["apod"], args["win_par"],
+        w.get("err_level", "stop"))
+
+
+def test_c1_fp64(gpu, oracle):
+    import workloads
+    w = workloads.c1()
+    ref, err, oob = oracle.dream_arr_rect(w["Ro"], w["a"], w["b"], w["dx"], w["dy"], w["dt"],
+                                          w["nt"], w["delay"], w["v"], w["cp"], w["G"],
+                                          err_level="stop")
+    assert err == 0 and oob == 0
+    H = _gpu_arr_rect(w)
+    assert H.shape == ref.shape and H.flags.f_contiguous
+    assert np.max(np.abs(H - ref)) <= 1e-12 * np.abs(ref).max()
+    assert np.array_equal(H != 0, ref != 0)
+
+
+@pytest.mark.parametrize("opt", [
+    dict(foc_met="x", focal=12.0), dict(foc_met="y", focal=9.0), dict(foc_met="xy", focal=15.0),
+    dict(foc_met="x+y", focal=15.0), dict(foc_met="ud", focal="ud"),
+    dict(steer_met="x", steer_par=[8.0, 0.0]), dict(steer_met="y", steer_par=[0.0, -5.0]),
+    dict(steer_met="xy", steer_par=[4.0, 3.0]),
+    dict(apod_met="ud"), dict(apod_met="triangle"), dict(apod_met="gauss"),
+    dict(apod_met="raised"), dict(apod_met="hann"), dict(apod_met="hamming"),
+    dict(apod_met="simply"), dict(apod_met="clamped"),
+])
+def test_options_fp64(gpu, oracle, opt):
+    rng = np.random.default_rng(len(str(opt)))
+    L = 9
+    G = np.c_[np.linspace(-4, 4, L), rng.uniform(-0.5, 0.5, L), np.zeros(L)]
+    No = 40
+    Ro = np.c_[rng.uniform(-5, 5, No), rng.uniform(-3, 3, No), rng.uniform(8, 30, No)]
+    w = dict(Ro=Ro, a=0.6, b=5.0, dx=0.05, dy=0.05, dt=0.02, nt=1600, delay=2.0, v=1.0, cp=1500.0,
+             G=G, err_level="ignore")
+    kw = dict(opt)
+    if kw.get("focal") == "ud":
+        kw["focal"] = rng.uniform(0, 1, L)
+    if kw.get("apod_met") == "ud":
+        kw["apod"] = rng.uniform(0.2, 1.0, L)
+    o = dict(foc_met=kw.get("foc_met", "off"), focal=kw.get("focal", 0.0),
+             steer_met=kw.get("steer_met", "off"), theta=kw.get("steer_par", [0, 0])[0],
+             phi=kw.get("steer_par", [0, 0])[1], apod_met=kw.get("apod_met", "off"),
+             apod=kw.get("apod"), apod_par=0.0)
+    ref, err, _ = oracle.dream_arr_rect(Ro, w["a"], w["b"], w["dx"], w["dy"], w["dt"], w["nt"],
+                                        w["delay"], 1.0, w["cp"], G, err_level="ignore", **o)
+    H = _gpu_arr_rect(w, **kw)
+    assert np.max(np.abs(H - ref)) <= 1e-12 * np.abs(ref).max()
+
+
+def test_fp32_output_and_store_mode(gpu, oracle):
+    """Many points (one work item per point, column store) and FP32 sums."""
+    from dream_amd.torch_ops import arr_rect
+    rng = np.random.default_rng(12)
+    L, No = 24, 3000
+    G = np.c_[np.linspace(-6, 6, L), np.zeros(L), np.zeros(L)]
+    Ro = np.c_[rng.uniform(-8, 8, No), np.zeros(No), rng.uniform(5, 40, No)]
+    delay = np.linalg.norm(Ro, axis=1) * 1000 / 1500 - 8.0
+    args = (0.45, 8.0, 0.05, 0.05, 0.02, 1800)
+    ref, err, oob = oracle.dream_arr_rect(Ro, *args, delay, 1.0, 1500.0, G, apod_met="hann",
+                                          err_level="ignore")
+    Rt = torch.tensor(Ro.T.copy(), device="cuda")
+    dt_ = torch.tensor(delay, device="cuda")
+    H64, code, oob64 = arr_rect(Rt, *args, dt_, 1.0, 1500.0, G, apod_met="hann",
+                                err_level="ignore", out_dtype=torch.float64)
+    assert np.max(np.abs(H64.cpu().numpy().T - ref)) <= 1e-12 * np.abs(ref).max()
+    assert oob64 == oob
+    H32, _, _ = arr_rect(Rt, *args, dt_, 1.0, 1500.0, G, apod_met="hann", err_level="ignore",
+                         out_dtype=torch.float32)
+    assert rel_l2(H32.double().cpu().numpy().T, ref) <= 2e-5
+
+
+def test_long_sir_global_mode(gpu, oracle):
+    """nt beyond the LDS histogram: accumulation straight into global memory."""
+    from dream_amd import dreamrect
+    Ro = np.array([[0.3, 0.2, 10.0], [1.0, -1.0, 60.0]])
+    args = (1.0, 6.0, 0.05, 0.05, 0.005)
+    nt = 12000
+    ref, _, _ = oracle.dreamrect(Ro, *args, nt, 0.0, 1.0, 1500.0, "stop")
+    H = dreamrect.dreamrect(Ro, [1.0, 6.0], [0.05, 0.05, 0.005, nt], 0.0, [1.0, 1500.0, 0.0],
+                            "stop")
+    assert np.max(np.abs(H - ref)) <= 1e-12 * np.abs(ref).max()
+
+
+def test_error_codes_gpu(gpu):
+    from dream_amd import dream_arr_rect, dreamrect
+    Ro = np.array([[0.0, 0.0, 20.0], [0.0, 0.0, 21.0], [0.0, 0.0, 22.0]])
+    with pytest.raises(RuntimeError):
+        dreamrect.dreamrect(Ro, [0.9, 10.0], [0.05, 0.05, 0.025, 10], 0.0, [1.0, 1500.0, 0.0], "stop")
+    H = dreamrect.dreamrect(Ro, [0.9, 10.0], [0.05, 0.05, 0.025, 10], 0.0, [1.0, 1500.0, 0.0],
+                            "ignore")
+    assert np.all(H == 0)
+    # The last-element rule: OOB only in the first element -> status none.
+    from dream_amd.torch_ops import arr_rect
+    Rt = torch.tensor(Ro.T.copy(), device="cuda")
+    delay = 20.0 / 1.5 - 1.0
+    G_first = np.array([[40.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    _, c1, n1 = arr_rect(Rt, 0.9, 10.0, 0.05, 0.05, 0.025, 200, delay, 1.0, 1500.0, G_first,
+                         err_level="ignore")
+    _, c2, n2 = arr_rect(Rt, 0.9, 10.0, 0.05, 0.05, 0.025, 200, delay, 1.0, 1500.0,
+                         G_first[::-1].copy(), err_level="ignore")
+    assert n1 > 0 and n1 == n2 and c1 == 0 and c2 == 1
+    _, c3, _ = arr_rect(Rt, 0.9, 10.0, 0.05, 0.05, 0.025, 200, delay, 1.0, 1500.0, G_first,
+                        err_level="stop")
+    assert c3 == 3
+    with pytest.raises(RuntimeError):
+        dream_arr_rect.dream_arr_rect(Ro, [0.9, 10.0], np.zeros((2, 3)), [0.05, 0.05, 0.025, 100],
+                                      0.0, [1.0, 1500.0, 5.0], "off", 0.0, "off", [0, 0], "off",
+                                      None, 0.0, "stop")   # attenuation: not on this path
+
+
+def test_batch_plan_uneven_with_graph_replays(gpu, oracle):
+    import workloads
+    from dream_amd.torch_ops import SirBatch
+    probs = workloads.c3(seed=33, n_problems=5)
+    # Shrink for the oracle: at most 300 points each.
+    for p in probs:
+        keep = min(300, p["Ro"].shape[0])
+        p["Ro"] = p["Ro"][:keep]
+        p["delay"] = p["delay"][:keep]
+    dev = []
+    for p in probs:
+        q = dict(p)
+        q["Ro"] = torch.tensor(p["Ro"].T.copy(), device="cuda")
+        q["delay"] = torch.tensor(p["delay"], device="cuda")
+        dev.append(q)
+    batch = SirBatch(dev, out_dtype=torch.float64)
+    refs = []
+    for p in probs:
+        ref, err, oob = oracle.dream_arr_rect(p["Ro"], p["a"], p["b"], p["dx"], p["dy"], p["dt"],
+                                              p["nt"], p["delay"], 1.0, p["cp"], p["G"],
+                                              foc_met="ud", focal=p["focal"], apod_met="hann",
+                                              err_level="ignore", nthreads=4)
+        refs.append((ref, oob))
+    for rep in range(3):
+        Hs = batch.run(use_graph=True)
+        torch.cuda.synchronize()
+        codes, oobs = batch.status()
+        for H, (ref, oob), o in zip(Hs, refs, oobs):
+            assert np.max(np.abs(H.cpu().numpy().T - ref)) <= 1e-12 * np.abs(ref).max(), rep
+            assert o == oob
+        torch.cuda.synchronize()
