@@ -148,7 +148,7 @@ int dream_hip_arr_rect_dev(const double *Ro, int64_t No, double a, double b, dou
                            int *sir_err, int64_t *oob_count, void *stream)
 {
   return guarded([&]() {
-    hipStream_t s = stream ? (hipStream_t) stream : lib_stream();
+    hipStream_t s = (hipStream_t) stream;   // NULL: the default stream
     std::vector<SirProblem> probs;
     probs.push_back(make_problem(Ro, No, a, b, dx, dy, dt, nt, delay_type, delay, v, cp,
                                  num_elements, G, foc_met, focal, steer_met, theta, phi, apod,
@@ -237,7 +237,7 @@ int dream_hip_sir_plan_run(void *plan, void *stream, int use_graph)
 {
   return guarded([&]() {
     require(plan != nullptr, "sir_plan_run: null plan");
-    static_cast<SirPlan *>(plan)->run(stream ? (hipStream_t) stream : lib_stream(), use_graph != 0);
+    static_cast<SirPlan *>(plan)->run((hipStream_t) stream, use_graph != 0);
     return DREAM_HIP_OK;
   });
 }
@@ -313,7 +313,7 @@ int dream_hip_das_dev(void *handle, const void *Y, const void *Ro, const void *G
   return guarded([&]() {
     require(handle != nullptr, "das: null handle");
     require(err_level >= 0 && err_level <= 4, "das: unknown error level");
-    hipStream_t s = stream ? (hipStream_t) stream : lib_stream();
+    hipStream_t s = (hipStream_t) stream;   // NULL: the default stream
     int code = static_cast<DasHandle *>(handle)->run_dev(
         Y, Ro, Gt, Gr, dt, delay_type, delay, cp, apod_t, apod_r, interp, n_frames,
         y_frame_stride, im_frame_stride, Im, err_level, sir_err != nullptr, s);
@@ -331,7 +331,7 @@ int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const
 {
   return guarded([&]() {
     require(handle != nullptr, "das adjoint: null handle");
-    hipStream_t s = stream ? (hipStream_t) stream : lib_stream();
+    hipStream_t s = (hipStream_t) stream;   // NULL: the default stream
     static_cast<DasHandle *>(handle)->adjoint_dev(Y, Ro, Gt, Gr, dt, delay, cp, apod_t, apod_r,
                                                   interp, n_frames, y_frame_stride,
                                                   im_frame_stride, g, gY, g_apod_t, g_apod_r,

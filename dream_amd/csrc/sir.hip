@@ -558,15 +558,19 @@ void SirPlan::run(hipStream_t s, bool use_graph)
 {
   if (cancelled()) throw Error(DREAM_HIP_ECANCELLED, "SIR cancelled");
   last_stream_ = s;
+  ran_ = true;
   if (!use_graph) {
     launch(s);
     return;
   }
   if (!graph_) {
+    // Capture on the library's own stream (the default stream cannot be
+    // captured); the graph is then launched on the caller's stream.
+    hipStream_t cs = lib_stream();
     hipGraph_t g = nullptr;
-    DREAM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    launch(s);
-    DREAM_HIP_CHECK(hipStreamEndCapture(s, &g));
+    DREAM_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    launch(cs);
+    DREAM_HIP_CHECK(hipStreamEndCapture(cs, &g));
     DREAM_HIP_CHECK(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
     DREAM_HIP_CHECK(hipGraphDestroy(g));
   }
@@ -577,7 +581,7 @@ void SirPlan::status(int32_t *codes, int64_t *oob)
 {
   const size_t P = host_.size();
   std::vector<Status> st(P);
-  if (last_stream_) DREAM_HIP_CHECK(hipStreamSynchronize(last_stream_));
+  if (ran_) DREAM_HIP_CHECK(hipStreamSynchronize(last_stream_));
   if (P) DREAM_HIP_CHECK(hipMemcpy(st.data(), d_status_, P * sizeof(Status), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < P; i++) {
     const int lv = levels_[i];

@@ -77,6 +77,7 @@ private:
   bool persistent_ = false;
   hipGraphExec_t graph_ = nullptr;
   hipStream_t last_stream_ = nullptr;
+  bool ran_ = false;
 };
 
 // Device parity helper: per-contribution sample indices of one

@@ -27,9 +27,9 @@
  *     on an argument or HIP runtime error; dream_hip_last_error() then holds
  *     the message (per thread).  The SIR/DAS out-of-bounds result is the
  *     separate SIRError written to *sir_err.
- *   - "_dev" entry points take device pointers for the large arrays and an
- *     optional hipStream_t (as void*; NULL = the library's stream) and do not
- *     synchronise; the other entry points take caller-owned host memory,
+ *   - "_dev" entry points take device pointers for the large arrays and a
+ *     hipStream_t (as void*; NULL = the default stream) and do not
+ *     synchronise unless noted; the other entry points take caller-owned host memory,
  *     copy it to the device, and return after the result is back in host
  *     memory, as the reference's CPU drivers do.
  *
@@ -114,9 +114,10 @@ int dream_hip_rect(double alpha, const double *Ro, int64_t No, double a, double 
  * (1 or No FP64) and h are DEVICE pointers; G, focal and apod are HOST
  * pointers (per-element delays and weights are evaluated on the host with
  * the C library's sin/cos/exp, as the reference does).  out_dtype is
- * DREAM_HIP_F64 or DREAM_HIP_F32 (FP32 sums, FP64 indices).  Asynchronous
- * on `stream`; *sir_err and *oob_count (may be NULL) are written once the
- * stream reaches the end of the call, so read them after synchronising. */
+ * DREAM_HIP_F64 or DREAM_HIP_F32 (FP32 sums, FP64 indices).  Runs on
+ * `stream` and returns after synchronising it, with *sir_err and *oob_count
+ * (either may be NULL) written.  (The plan API below is the asynchronous,
+ * graph-capturable form.) */
 int dream_hip_arr_rect_dev(const double *Ro, int64_t No, double a, double b, double dx,
                            double dy, double dt, int64_t nt, int delay_type,
                            const double *delay, double v, double cp, int64_t num_elements,
