@@ -55,7 +55,7 @@ def cpu_threads():
         return os.cpu_count() or 1
 
 
-def bench_c2(steps, warmup, want_cpu):
+def bench_c2(steps, warmup, want_cpu, want_extra=True):
     import numpy as np
     import torch
 
@@ -81,7 +81,7 @@ def bench_c2(steps, warmup, want_cpu):
 
     extra = {}
     # FP64 and the reference-float path on the same image.
-    for prec, dtype in (("f64", torch.float64), ("f32_ref", torch.float32)):
+    for prec, dtype in ((("f64", torch.float64), ("f32_ref", torch.float32)) if want_extra else ()):
         p = DasPlan("tfm", w["K"], w["Ro"], w["Gt"], w["Gr"], precision=prec)
         Yp = Y.to(dtype)
         o = torch.empty(n_pix, dtype=dtype, device="cuda")
@@ -122,7 +122,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: headline kernel only, plus a 256 MiB "
+                         "device-to-device copy used to calibrate the HBM byte counters")
     args = ap.parse_args()
+    if args.profile:
+        args.no_cpu = True
+        args.no_extra = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); run with --gpus 1",
               file=sys.stderr)
@@ -132,7 +138,15 @@ def main():
     from dream_amd import _lib
     _lib.require_device()
 
-    res, extra, cpu = bench_c2(args.steps, args.warmup, not args.no_cpu)
+    res, extra, cpu = bench_c2(args.steps, args.warmup, not args.no_cpu, not args.no_extra)
+    if args.profile:
+        # Calibration: a plain copy of 256 MiB (256 MiB read + 256 MiB written).
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x 268435456 bytes")
     line = {
         "metric": METRIC,
         "value": res["value"],

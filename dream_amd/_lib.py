@@ -31,6 +31,7 @@ EXPORTS = (
     "dream_hip_rect_indices", "dream_hip_sir_plan_create", "dream_hip_sir_plan_run",
     "dream_hip_sir_plan_status", "dream_hip_sir_plan_destroy",
     "dream_hip_das_create", "dream_hip_das_destroy", "dream_hip_das_setup_has_changed",
+    "dream_hip_das_set_pixel_order", "dream_hip_pixel_order",
     "dream_hip_das", "dream_hip_das_dev", "dream_hip_das_adjoint_dev",
 )
 
@@ -102,6 +103,10 @@ def _declare(L):
     L.dream_hip_das_destroy.argtypes = [_vp]
     L.dream_hip_das_setup_has_changed.restype = _i32
     L.dream_hip_das_setup_has_changed.argtypes = [_vp, _i32, _i64, _i64, _i64, _i64, _i32]
+    L.dream_hip_das_set_pixel_order.restype = _i32
+    L.dream_hip_das_set_pixel_order.argtypes = [_vp, _i32p, _i64]
+    L.dream_hip_pixel_order.restype = _i32
+    L.dream_hip_pixel_order.argtypes = [_vp, _i64, _i32p]
     L.dream_hip_das.restype = _i32
     L.dream_hip_das.argtypes = [_vp, _vp, _vp, _vp, _vp, _dbl, _i32, _vp, _dbl, _vp, _i32, _ip]
     L.dream_hip_das_dev.restype = _i32

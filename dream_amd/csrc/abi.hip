@@ -290,6 +290,24 @@ int dream_hip_das_setup_has_changed(void *handle, int das_type, int64_t a_scan_l
              : 0;
 }
 
+int dream_hip_das_set_pixel_order(void *handle, const int32_t *perm, int64_t n)
+{
+  return guarded([&]() {
+    require(handle != nullptr, "das: null handle");
+    static_cast<DasHandle *>(handle)->set_pixel_order(perm, n);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_pixel_order(const double *Ro, int64_t No, int32_t *perm)
+{
+  return guarded([&]() {
+    require(No >= 0 && (No == 0 || (Ro && perm)), "pixel_order: bad arguments");
+    pixel_order(Ro, No, perm);
+    return DREAM_HIP_OK;
+  });
+}
+
 int dream_hip_das(void *handle, const void *Y, const void *Ro, const void *Gt, const void *Gr,
                   double dt, int delay_type, const void *delay, double cp, void *Im,
                   int err_level, int *sir_err)

@@ -31,6 +31,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 #include "das.hpp"
 
@@ -78,114 +80,171 @@ __device__ __forceinline__ double wave_sum_d(double v)
   return v;
 }
 
-// One pair's sample: nearest-lower (reference) or linear.  Accumulates
-// w*value into acc, returns whether the nearest-lower index was in range.
-template <typename DT, typename GT, int INTERP>
-__device__ __forceinline__ bool sample(const DT *__restrict__ y, int K, GT tdp, GT fs, DT w,
-                                       DT &acc)
-{
-  if (INTERP == DREAM_HIP_NEAREST) {
-    const GT x = tdp * fs;
-    if (x > GT(-1) && x < (GT) K) {
-      acc += w * y[(int) x];
-      return true;
-    }
-    return false;
-  } else {
-    const GT u = tdp * fs;
-    const GT fk = floor(u);
-    if (fk > GT(-2) && fk < (GT) K) {
-      const int k = (int) fk;
-      const DT f = (DT) (u - fk);
-      const DT y0 = k >= 0 ? y[k] : DT(0);
-      const DT y1 = k + 1 < K ? y[k + 1] : DT(0);
-      acc += w * (y0 + f * (y1 - y0));
-    }
-    return fk >= GT(0) && fk < (GT) K;
-  }
-}
-
-// Value and slope (y[k+1]-y[k]) of one pair for the adjoint; k via the
-// chosen rule, f the interpolation fraction (0 for nearest).
-template <typename DT, typename GT, int INTERP>
-__device__ __forceinline__ void sample2(const DT *__restrict__ y, int K, GT u, DT &val, DT &slope,
-                                        int &k, DT &f)
-{
-  if (INTERP == DREAM_HIP_NEAREST) {
-    f = DT(0);
-    slope = DT(0);
-    if (u > GT(-1) && u < (GT) K) {
-      k = (int) u;
-      val = y[k];
-    } else {
-      k = -2;
-      val = DT(0);
-    }
-  } else {
-    const GT fk = floor(u);
-    if (fk > GT(-2) && fk < (GT) K) {
-      k = (int) fk;
-      f = (DT) (u - fk);
-      const DT y0 = k >= 0 ? y[k] : DT(0);
-      const DT y1 = k + 1 < K ? y[k + 1] : DT(0);
-      slope = y1 - y0;
-      val = y0 + f * slope;
-    } else {
-      k = -2;
-      f = DT(0);
-      val = DT(0);
-      slope = DT(0);
-    }
-  }
-}
-
-struct PixelFrame {
-  int64_t p;       // pixel index in Ro / Im
-  bool valid;
-};
-
 __device__ __forceinline__ int64_t pixel_of(const DasArgs &A, const int32_t *perm, int64_t idx)
 {
   return perm ? (int64_t) perm[idx] : idx;
 }
 
+// --------------------------------------------------------------------- gathers
+
+// A-scan samples are read with raw buffer loads over one frame of Y: an
+// offset past the frame returns 0 in hardware without touching memory, so
+// every gather of an unrolled block is issued unconditionally and the block
+// keeps all its loads in flight (one wait per block instead of one per pair).
+constexpr unsigned kNoSample = 0xFFFFFFF0u;   // byte offset beyond any frame
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(const void *base, int64_t bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short) 0, (int) bytes,
+                                           0x00020000);
+}
+
+template <typename DT>
+__device__ __forceinline__ DT bload(__amdgpu_buffer_rsrc_t r, unsigned byte_off);
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned byte_off)
+{
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int) byte_off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, unsigned byte_off)
+{
+  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int) byte_off, 0, 0);
+  return __hiloint2double((int) v[1], (int) v[0]);
+}
+
+// Legs are clamped to |t| <= leg_max = 2^26 / fs before use, so every sample
+// position u = (t_t + t_r) * fs lies in [-2^27, 2^27] and converts to int
+// exactly as trunc/floor would; a NaN leg becomes the clamp value and lands
+// out of bounds, as in the reference.  Only legs that are out of bounds
+// anyway are changed.
 template <typename GT>
-struct Stripes {
-  GT y_lo, y_hi, x_lo, x_hi;
-};
+__device__ __forceinline__ GT clamp_leg(GT t, GT tmax)
+{
+  return fmin(fmax(t, -tmax), tmax);
+}
+
+// One pair: the value of A-scan `abase` (element offset a*K) at sample
+// position u under the chosen rule; `inb` reports whether the nearest-lower
+// index lies in [0, K) (the reference's bounds test).
+template <typename DT, typename GT, int INTERP>
+__device__ __forceinline__ DT fetch(__amdgpu_buffer_rsrc_t r, unsigned abase, unsigned K, GT u,
+                                    bool live, bool &inb)
+{
+  if (INTERP == DREAM_HIP_NEAREST) {
+    const int k = (int) u;                       // trunc toward zero, u in [-2^27, 2^27]
+    inb = (unsigned) k < K;
+    const unsigned off = (inb && live) ? (abase + (unsigned) k) * (unsigned) sizeof(DT) : kNoSample;
+    return bload<DT>(r, off);
+  } else {
+    const GT fk = floor(u);
+    const int k = (int) fk;
+    inb = (unsigned) k < K;
+    const bool in1 = (unsigned) (k + 1) < K;
+    const unsigned o0 = (inb && live) ? (abase + (unsigned) k) * (unsigned) sizeof(DT) : kNoSample;
+    const unsigned o1 = (in1 && live) ? (abase + (unsigned) (k + 1)) * (unsigned) sizeof(DT) : kNoSample;
+    const DT y0 = bload<DT>(r, o0);
+    const DT y1 = bload<DT>(r, o1);
+    const DT f = (DT) (u - fk);
+    return y0 + f * (y1 - y0);
+  }
+}
+
+// Forward gathers use one buffer descriptor per A-scan (base = the A-scan's
+// first sample, size = K samples): the hardware returns 0 for any sample
+// index outside [0, K), so a gather is a shift and a load, with no compare.
+// Legs are clamped (leg_max = 2^26 / fs) so |u| <= 2^27 and a negative
+// index times sizeof(DT) never wraps back into [0, K).
+template <typename DT>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ascan_rsrc(const DT *ascan, unsigned kbytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<DT *>(ascan), (short) 0, (int) kbytes,
+                                           0x00020000);
+}
+
+template <typename DT, typename GT, int INTERP>
+__device__ __forceinline__ DT fetch_ascan(const DT *ascan, unsigned kbytes, GT u, int &k)
+{
+  const __amdgpu_buffer_rsrc_t r = ascan_rsrc(ascan, kbytes);
+  if (INTERP == DREAM_HIP_NEAREST) {
+    k = (int) u;                                // trunc toward zero
+    return bload<DT>(r, (unsigned) k * (unsigned) sizeof(DT));
+  } else {
+    const GT fk = floor(u);
+    k = (int) fk;
+    const DT y0 = bload<DT>(r, (unsigned) k * (unsigned) sizeof(DT));
+    const DT y1 = bload<DT>(r, (unsigned) (k + 1) * (unsigned) sizeof(DT));
+    const DT f = (DT) (u - fk);
+    return y0 + f * (y1 - y0);
+  }
+}
 
 // --------------------------------------------------------------------- forward
 
-template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
-__global__ void __launch_bounds__(kThreads)
-das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm)
+// Work of a 64-pixel tile is cut into jobs: (receive chunk of kRC, transmit
+// block of kTB) for two-leg modes, blocks of kSB elements for SAFT.  The 4
+// waves of a work-group share the tile and take jobs round-robin; `split`
+// work-groups (gridDim.y) share a tile when the tiles alone cannot fill the
+// device, writing partial sums that das_reduce_kernel adds in fixed order.
+constexpr int kTB = 16;    // transmit elements per two-leg job
+constexpr int kSB = 8;     // SAFT elements per job (one unrolled block)
+
+constexpr int kMaxWaves = 8;     // waves per forward work-group (64 .. 512 threads)
+
+template <typename DT, typename GT, int TYPE, int INTERP, bool APOD, bool STATUS>
+__global__ void __launch_bounds__(64 * kMaxWaves)
+das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__restrict__ partial)
 {
-  const int64_t idx = (int64_t) blockIdx.x * kThreads + threadIdx.x;
-  const int64_t frame = blockIdx.y;
+  __shared__ DT red[kMaxWaves][64];
+  const int nwaves = blockDim.x >> 6;
+  // The wave index is uniform across the wave; say so, so that job, element
+  // and A-scan indices (and the per-A-scan descriptors) live in SGPRs.
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t idx = (int64_t) blockIdx.x * 64 + lane;
+  const int sidx = blockIdx.y;
+  const int64_t frame = blockIdx.z;
   const bool valid = idx < A.No;
   const int64_t p = valid ? pixel_of(A, perm, idx) : 0;
   const GT *Ro = (const GT *) A.Ro;
   const GT xo = Ro[p], yo = Ro[p + A.No], zo = Ro[p + 2 * A.No];
   const GT *gdel = (const GT *) A.delay;
-  const GT dl = A.delay_multi ? delay_ms_of<GT>(gdel[p]) : (GT) A.delay_ms;
+  const GT dl = delay_ms_of<GT>(gdel[A.delay_multi ? p : 0]);
   const GT fs = (GT) A.fs_khz, inv_cp = (GT) A.inv_cp;
-  const DT *Y = (const DT *) A.Y + frame * A.y_stride;
+  const GT tmax = (GT) A.leg_max;
   const int K = (int) A.K;
+  const unsigned Ku = (unsigned) K;
+  const unsigned kbytes = Ku * (unsigned) sizeof(DT);
+  const DT *Yf = (const DT *) A.Y + frame * A.y_stride;
   const GT *Gt = (const GT *) A.Gt;
   const GT *Gr = (const GT *) A.Gr;
   const DT *at = (const DT *) A.apod_t;
   const DT *ar = (const DT *) A.apod_r;
   const int Lt = A.Lt, Lr = A.Lr;
+  const int stride = nwaves * split;
   DT acc = DT(0);
   bool oob = false;
 
   if (TYPE == DAS_SAFT) {
-    for (int n = 0; n < Lt; n++) {
-      GT t = leg3<GT>(Gt[n] - xo, Gt[n + Lt] - yo, Gt[n + 2 * Lt] - zo) * inv_cp;
-      t += dl;
-      const GT tdp = t + t;                       // (T)(2.0*t), exact
-      const DT w = APOD ? at[n] : DT(1);
-      oob |= !sample<DT, GT, INTERP>(Y + (int64_t) n * K, K, tdp, fs, w, acc);
+    const int J = (Lt + kSB - 1) / kSB;
+    for (int job = sidx * nwaves + wave; job < J; job += stride) {
+      const int n0 = job * kSB;
+      DT v[kSB];
+#pragma unroll
+      for (int q = 0; q < kSB; q++) {
+        const int n = min(n0 + q, Lt - 1);
+        const bool live = n0 + q < Lt;
+        GT t = leg3<GT>(Gt[n] - xo, Gt[n + Lt] - yo, Gt[n + 2 * Lt] - zo) * inv_cp;
+        t += dl;
+        const GT tdp = clamp_leg<GT>(t + t, tmax);  // (T)(2.0*t), exact
+        int k;
+        v[q] = fetch_ascan<DT, GT, INTERP>(Yf + (int64_t) n * K, live ? kbytes : 0u, tdp * fs, k);
+        if (STATUS) oob |= live && (unsigned) k >= Ku;
+      }
+      DT part = DT(0);
+#pragma unroll
+      for (int q = 0; q < kSB; q++) part += APOD ? at[min(n0 + q, Lt - 1)] * v[q] : v[q];
+      acc += part;
     }
   } else {
     GT gy_c = GT(0), gx_r = GT(0);
@@ -193,7 +252,12 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm)
       gy_c = stripe_gap<GT>(yo, Gr[Lr], Gr[2 * Lr - 1]);
       gx_r = stripe_gap<GT>(xo, Gt[0], Gt[Lt - 1]);
     }
-    for (int j0 = 0; j0 < Lr; j0 += kRC) {
+    const int nchunks = (Lr + kRC - 1) / kRC;
+    const int J = nchunks * ((Lt + kTB - 1) / kTB);
+    for (int job = sidx * nwaves + wave; job < J; job += stride) {
+      const int c = job % nchunks, b = job / nchunks;
+      const int j0 = c * kRC, i0 = b * kTB, i1 = min(i0 + kTB, Lt);
+      const bool full = j0 + kRC <= Lr;
       GT tr[kRC];
 #pragma unroll
       for (int jj = 0; jj < kRC; jj++) {
@@ -202,51 +266,109 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm)
           tr[jj] = leg3<GT>(Gr[j] - xo, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
         else
           tr[jj] = leg3<GT>(gx_r, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
+        tr[jj] = clamp_leg<GT>(tr[jj], tmax);
       }
-      const int nj = min(kRC, Lr - j0);
-      for (int i = 0; i < Lt; i++) {
+      for (int i = i0; i < i1; i++) {
         GT tt;
         if (TYPE == DAS_TFM)
           tt = leg3<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
         else
           tt = leg3<GT>(Gt[i] - xo, gy_c, Gt[i + 2 * Lt] - zo) * inv_cp;
         tt += dl;
+        tt = clamp_leg<GT>(tt, tmax);
         const DT wi = APOD ? at[i] : DT(1);
-        DT part = DT(0);
-        if (nj == kRC) {
+        // All kRC gathers of the block are issued before any is consumed.
+        DT v[kRC];
+        if (full) {
 #pragma unroll
           for (int jj = 0; jj < kRC; jj++) {
             const int j = j0 + jj;
-            const int64_t a = (TYPE == DAS_RCA_ROWTX) ? (int64_t) j * Lt + i : (int64_t) i * Lr + j;
-            const DT w = APOD ? wi * ar[j] : DT(1);
-            oob |= !sample<DT, GT, INTERP>(Y + a * K, K, tt + tr[jj], fs, w, part);
+            const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
+            int k;
+            v[jj] = fetch_ascan<DT, GT, INTERP>(Yf + (int64_t) a * K, kbytes, (tt + tr[jj]) * fs, k);
+            if (STATUS) oob |= (unsigned) k >= Ku;
           }
         } else {
-          for (int jj = 0; jj < nj; jj++) {
-            const int j = j0 + jj;
-            const int64_t a = (TYPE == DAS_RCA_ROWTX) ? (int64_t) j * Lt + i : (int64_t) i * Lr + j;
-            const DT w = APOD ? wi * ar[j] : DT(1);
-            GT trj = tr[0];
 #pragma unroll
-            for (int q = 1; q < kRC; q++) if (q == jj) trj = tr[q];
-            oob |= !sample<DT, GT, INTERP>(Y + a * K, K, tt + trj, fs, w, part);
+          for (int jj = 0; jj < kRC; jj++) {
+            const int j = min(j0 + jj, Lr - 1);
+            const bool live = j0 + jj < Lr;
+            const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
+            int k;
+            v[jj] = fetch_ascan<DT, GT, INTERP>(Yf + (int64_t) a * K, live ? kbytes : 0u,
+                                                (tt + tr[jj]) * fs, k);
+            if (STATUS) oob |= live && (unsigned) k >= Ku;
           }
         }
-        acc += part;
+        DT part = DT(0);
+#pragma unroll
+        for (int jj = 0; jj < kRC; jj++) part += APOD ? ar[min(j0 + jj, Lr - 1)] * v[jj] : v[jj];
+        acc += APOD ? wi * part : part;
       }
     }
   }
-  if (valid) ((DT *) A.Im)[frame * A.im_stride + p] = acc;
-  oob = oob && valid;
-  if (__any(oob) && (threadIdx.x & 63) == 0) atomicOr(&A.status->any_oob, 1u);
+  // Sum the waves' partial sums in a fixed order.
+  red[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0 && valid) {
+    DT sum = red[0][lane];
+    for (int w = 1; w < nwaves; w++) sum += red[w][lane];
+    if (split == 1) ((DT *) A.Im)[frame * A.im_stride + p] = sum;
+    else partial[((int64_t) sidx * A.n_frames + frame) * A.No + p] = sum;
+  }
+  if (STATUS) {
+    oob = oob && valid;
+    if (__any(oob) && lane == 0) atomicOr(&A.status->any_oob, 1u);
+  }
 }
 
-// --------------------------------------------------------------------- adjoint
+// Im[f][p] = sum over s of partial[s][f][p], s ascending (split launches).
+template <typename DT>
+__global__ void das_reduce_kernel(const DT *__restrict__ partial, int split, int64_t No,
+                                  int64_t n_frames, int64_t im_stride, DT *__restrict__ Im)
+{
+  const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= No * n_frames) return;
+  const int64_t f = q / No, p = q - f * No;
+  DT sum = partial[q];
+  for (int s = 1; s < split; s++) sum += partial[(int64_t) s * n_frames * No + q];
+  Im[f * im_stride + p] = sum;
+}
+
+// Both taps of one pair for the adjoint: y0 = Y[k], y1 = Y[k+1] (0 outside
+// [0, K)), the fraction f, and k (-2 when neither tap exists).  The nearest
+// rule uses k = trunc(u), y1 = y0 and f = 0.  Branch-free: loads are issued
+// unconditionally through the frame's buffer resource.
+template <typename DT, typename GT, int INTERP>
+__device__ __forceinline__ void fetch_taps(__amdgpu_buffer_rsrc_t r, unsigned abase, int K, GT u,
+                                           bool live, DT &y0, DT &y1, DT &f, int &k)
+{
+  if (INTERP == DREAM_HIP_NEAREST) {
+    const int kk = (int) u;                      // u in [-2^27, 2^27]
+    const bool inb = live && (unsigned) kk < (unsigned) K;
+    k = inb ? kk : -2;
+    y0 = bload<DT>(r, inb ? (abase + (unsigned) kk) * (unsigned) sizeof(DT) : kNoSample);
+    y1 = y0;
+    f = DT(0);
+  } else {
+    const GT fk = floor(u);
+    const int kk = (int) fk;
+    const bool t0 = live && (unsigned) kk < (unsigned) K;
+    const bool t1 = live && (unsigned) (kk + 1) < (unsigned) K;
+    y0 = bload<DT>(r, t0 ? (abase + (unsigned) kk) * (unsigned) sizeof(DT) : kNoSample);
+    y1 = bload<DT>(r, t1 ? (abase + (unsigned) (kk + 1)) * (unsigned) sizeof(DT) : kNoSample);
+    f = (t0 || t1) ? (DT) (u - fk) : DT(0);
+    k = (t0 || t1) ? kk : -2;
+  }
+}
 
 // Channel-major adjoint: one work-group per (pixel block, A-scan, frame).
 // Scatters the A-scan's Y-gradient into LDS, then adds the touched window
 // into gY; accumulates sum_p g_p*value (z_t[a]) and sum_p g_p*w*slope*du/dcp
-// (z_cp) for the A-scan.
+// (z_cp) for the A-scan.  Each lane handles kPU pixels per step so that
+// their 2 kPU gathers are in flight together.
+constexpr int kPU = 4;
+
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
 __global__ void __launch_bounds__(kThreads)
 das_adj_channel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t pix_per_block,
@@ -274,8 +396,12 @@ das_adj_channel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t pix_
   const GT *Ro = (const GT *) A.Ro;
   const GT *Gt = (const GT *) A.Gt;
   const GT *Gr = (const GT *) A.Gr;
-  const GT fs = (GT) A.fs_khz, inv_cp = (GT) A.inv_cp, dl = (GT) A.delay_ms, cp = (GT) A.cp;
-  const DT *y = (const DT *) A.Y + frame * A.y_stride + a * K;
+  const GT fs = (GT) A.fs_khz, inv_cp = (GT) A.inv_cp, dl = (GT) A.delay_ms;
+  const double dudc_scale = -(double) A.fs_khz / A.cp;    // du/dcp = -(travel time) fs / cp
+  const GT tmax = (GT) A.leg_max;
+  const __amdgpu_buffer_rsrc_t ry =
+      frame_rsrc((const DT *) A.Y + frame * A.y_stride, A.y_stride * (int64_t) sizeof(DT));
+  const unsigned abase = (unsigned) (a * K);
   const DT *g = (const DT *) A.g + frame * A.im_stride;
   const DT *at = (const DT *) A.apod_t;
   const DT *ar = (const DT *) A.apod_r;
@@ -288,46 +414,60 @@ das_adj_channel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t pix_
   const int64_t p1 = min(p0 + pix_per_block, A.No);
   double z = 0.0, zc = 0.0;
   int kmin = K, kmax = -1;
-  for (int64_t idx = p0 + threadIdx.x; idx < p1; idx += kThreads) {
-    const int64_t p = pixel_of(A, perm, idx);
-    const GT xo = Ro[p], yo = Ro[p + A.No], zo = Ro[p + 2 * A.No];
-    const DT gp = g[p];
-    GT u, tsum;   // tsum: total travel time without the delay, for du/dcp
-    if (TYPE == DAS_SAFT) {
-      GT t = leg3<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
-      tsum = t + t;
-      t += dl;
-      u = (t + t) * fs;
-    } else {
-      GT tt, tr;
-      if (TYPE == DAS_TFM) {
-        tt = leg3<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
-        tr = leg3<GT>(Gr[j] - xo, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
+  for (int64_t base = p0 + threadIdx.x; base < p1; base += (int64_t) kThreads * kPU) {
+    GT u[kPU], tsum[kPU];
+    DT gp[kPU];
+    bool live[kPU];
+#pragma unroll
+    for (int q = 0; q < kPU; q++) {
+      const int64_t idx = base + (int64_t) q * kThreads;
+      live[q] = idx < p1;
+      const int64_t p = pixel_of(A, perm, live[q] ? idx : p0);
+      const GT xo = Ro[p], yo = Ro[p + A.No], zo = Ro[p + 2 * A.No];
+      gp[q] = live[q] ? g[p] : DT(0);
+      if (TYPE == DAS_SAFT) {
+        GT t = leg3<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
+        tsum[q] = t + t;
+        t += dl;
+        u[q] = clamp_leg<GT>(t + t, tmax) * fs;
       } else {
-        tt = leg3<GT>(Gt[i] - xo, stripe_gap<GT>(yo, y_lo, y_hi), Gt[i + 2 * Lt] - zo) * inv_cp;
-        tr = leg3<GT>(stripe_gap<GT>(xo, x_lo, x_hi), Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
+        GT tt, tr;
+        if (TYPE == DAS_TFM) {
+          tt = leg3<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
+          tr = leg3<GT>(Gr[j] - xo, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
+        } else {
+          tt = leg3<GT>(Gt[i] - xo, stripe_gap<GT>(yo, y_lo, y_hi), Gt[i + 2 * Lt] - zo) * inv_cp;
+          tr = leg3<GT>(stripe_gap<GT>(xo, x_lo, x_hi), Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
+        }
+        tsum[q] = tt + tr;
+        tt += dl;
+        u[q] = (clamp_leg<GT>(tt, tmax) + clamp_leg<GT>(tr, tmax)) * fs;
       }
-      tsum = tt + tr;
-      tt += dl;
-      u = (tt + tr) * fs;
     }
-    DT val, slope, f;
-    int k;
-    sample2<DT, GT, INTERP>(y, K, u, val, slope, k, f);
-    z += (double) (gp * val);
-    // d u / d cp = -(travel time without delay) * fs / cp
-    zc += (double) (gp * w * slope) * (-(double) tsum * (double) fs / (double) cp);
-    if (want_gy && k > -2) {
-      const DT gw = gp * w;
-      if (INTERP == DREAM_HIP_NEAREST) {
-        atomicAdd(&gacc[k], gw);
-        kmin = min(kmin, k);
-        kmax = max(kmax, k);
-      } else {
-        if (k >= 0) atomicAdd(&gacc[k], gw * (DT(1) - f));
-        if (k + 1 < K) atomicAdd(&gacc[k + 1], gw * f);
-        kmin = min(kmin, max(k, 0));
-        kmax = max(kmax, min(k + 1, K - 1));
+    DT y0[kPU], y1[kPU], f[kPU];
+    int k[kPU];
+#pragma unroll
+    for (int q = 0; q < kPU; q++)
+      fetch_taps<DT, GT, INTERP>(ry, abase, K, u[q], live[q], y0[q], y1[q], f[q], k[q]);
+#pragma unroll
+    for (int q = 0; q < kPU; q++) {
+      const DT slope = INTERP == DREAM_HIP_LINEAR ? y1[q] - y0[q] : DT(0);
+      const DT val = y0[q] + f[q] * slope;
+      z += (double) (gp[q] * val);
+      if (INTERP == DREAM_HIP_LINEAR)
+        zc += (double) (gp[q] * w * slope) * ((double) tsum[q] * dudc_scale);
+      if (want_gy && k[q] > -2) {
+        const DT gw = gp[q] * w;
+        if (INTERP == DREAM_HIP_NEAREST) {
+          atomicAdd(&gacc[k[q]], gw);
+          kmin = min(kmin, k[q]);
+          kmax = max(kmax, k[q]);
+        } else {
+          if (k[q] >= 0) atomicAdd(&gacc[k[q]], gw * (DT(1) - f[q]));
+          if (k[q] + 1 < K) atomicAdd(&gacc[k[q] + 1], gw * f[q]);
+          kmin = min(kmin, max(k[q], 0));
+          kmax = max(kmax, min(k[q] + 1, K - 1));
+        }
       }
     }
   }
@@ -352,11 +492,11 @@ das_adj_channel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t pix_
   if (want_gy) {
     DT *gy = (DT *) A.gY + frame * A.y_stride + a * K;
     if (blocks_per_channel == 1) {
-      for (int k = threadIdx.x; k < K; k += kThreads) gy[k] = gacc[k];
+      for (int kk = threadIdx.x; kk < K; kk += kThreads) gy[kk] = gacc[kk];
     } else {
-      for (int k = s_kmin + threadIdx.x; k <= s_kmax; k += kThreads) {
-        const DT v = gacc[k];
-        if (v != DT(0)) unsafeAtomicAdd(&gy[k], v);
+      for (int kk = s_kmin + threadIdx.x; kk <= s_kmax; kk += kThreads) {
+        const DT v = gacc[kk];
+        if (v != DT(0)) unsafeAtomicAdd(&gy[kk], v);
       }
     }
   }
@@ -365,7 +505,7 @@ das_adj_channel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t pix_
 // Pixel-major adjoint for two-leg modes without gY: accumulates
 // z_t[i] = sum_p g_p sum_j w_j v(p,i,j), z_r[j] = sum_p g_p sum_i w_i v(p,i,j)
 // and z_cp, in LDS per work-group, flushed once.  Grid-stride over
-// (frame, 256-pixel tile).
+// (frame, 256-pixel tile); gathers as in the forward kernel.
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
 __global__ void __launch_bounds__(kThreads)
 das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tiles)
@@ -387,6 +527,7 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
   const int K = (int) A.K;
   const int64_t tiles_per_frame = (A.No + kThreads - 1) / kThreads;
   const GT y_lo = Gr[Lr], y_hi = Gr[2 * Lr - 1], x_lo = Gt[0], x_hi = Gt[Lt - 1];
+  const GT tmax = (GT) A.leg_max;
   double zcp = 0.0;
   const bool lane0 = (threadIdx.x & 63) == 0;
 
@@ -397,7 +538,8 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
     const int64_t p = valid ? pixel_of(A, perm, idx) : 0;
     const GT xo = Ro[p], yo = Ro[p + A.No], zo = Ro[p + 2 * A.No];
     const DT gp = valid ? ((const DT *) A.g)[frame * A.im_stride + p] : DT(0);
-    const DT *Y = (const DT *) A.Y + frame * A.y_stride;
+    const __amdgpu_buffer_rsrc_t ry =
+        frame_rsrc((const DT *) A.Y + frame * A.y_stride, A.y_stride * (int64_t) sizeof(DT));
     GT gy_c = GT(0), gx_r = GT(0);
     if (TYPE != DAS_TFM) {
       gy_c = stripe_gap<GT>(yo, y_lo, y_hi);
@@ -415,7 +557,6 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
           tr[jj] = leg3<GT>(gx_r, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
         zj[jj] = DT(0);
       }
-      const int nj = min(kRC, Lr - j0);
       for (int i = 0; i < Lt; i++) {
         GT tt;
         if (TYPE == DAS_TFM)
@@ -425,21 +566,25 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
         const GT tt0 = tt;
         tt += dl;
         const DT wi = APOD ? at[i] : DT(1);
-        DT zi = DT(0);
-        DT zc = DT(0);
+        DT y0[kRC], y1[kRC], f[kRC];
+        int k[kRC];
 #pragma unroll
         for (int jj = 0; jj < kRC; jj++) {
-          if (jj < nj) {
-            const int j = j0 + jj;
-            const int64_t a = (TYPE == DAS_RCA_ROWTX) ? (int64_t) j * Lt + i : (int64_t) i * Lr + j;
-            const DT wj = APOD ? ar[j] : DT(1);
-            DT val, slope, f;
-            int k;
-            sample2<DT, GT, INTERP>(Y + a * K, K, (tt + tr[jj]) * fs, val, slope, k, f);
-            zi += wj * val;
-            zj[jj] += wi * val;
-            if (INTERP == DREAM_HIP_LINEAR) zc += wi * wj * slope * (DT) (tt0 + tr[jj]);
-          }
+          const int j = min(j0 + jj, Lr - 1);
+          const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
+          fetch_taps<DT, GT, INTERP>(ry, (unsigned) a * (unsigned) K, K,
+                                     (clamp_leg<GT>(tt, tmax) + clamp_leg<GT>(tr[jj], tmax)) * fs,
+                                     valid && j0 + jj < Lr, y0[jj], y1[jj], f[jj], k[jj]);
+        }
+        DT zi = DT(0), zc = DT(0);
+#pragma unroll
+        for (int jj = 0; jj < kRC; jj++) {
+          const DT wj = APOD ? ar[min(j0 + jj, Lr - 1)] : DT(1);
+          const DT slope = INTERP == DREAM_HIP_LINEAR ? y1[jj] - y0[jj] : DT(0);
+          const DT val = y0[jj] + f[jj] * slope;
+          zi += wj * val;
+          zj[jj] += wi * val;
+          if (INTERP == DREAM_HIP_LINEAR) zc += wi * wj * slope * (DT) (tt0 + tr[jj]);
         }
         const float zsum = wave_sum_f((float) (gp * zi));
         if (lane0 && zsum != 0.0f) atomicAdd(&zt[i], (double) zsum);
@@ -447,7 +592,7 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
       }
 #pragma unroll
       for (int jj = 0; jj < kRC; jj++) {
-        if (jj < nj) {
+        if (j0 + jj < Lr) {
           const float zsum = wave_sum_f((float) (gp * zj[jj]));
           if (lane0 && zsum != 0.0f) atomicAdd(&zr[j0 + jj], (double) zsum);
         }
@@ -535,36 +680,73 @@ int cu_count()
 
 // ------------------------------------------------------------------ dispatch
 
-template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
-void fwd_launch(const DasArgs &a, const int32_t *perm, hipStream_t s)
+// Launch shape of the forward kernel: `waves` waves per work-group share a
+// 64-pixel tile; `split` work-groups share a tile (partial sums + reduce
+// kernel) only when tiles x kMaxWaves waves cannot fill the device.
+struct FwdShape {
+  int waves, split;
+};
+
+FwdShape forward_shape(int type, int64_t No, int64_t n_frames, int64_t Lt, int64_t Lr)
 {
-  dim3 grid((unsigned) ((a.No + kThreads - 1) / kThreads), (unsigned) a.n_frames);
-  hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD>), grid, dim3(kThreads), 0, s,
-                     a, perm);
+  const int64_t tiles = ((No + 63) / 64) * n_frames;
+  const int64_t target = (int64_t) cu_count() * 32;      // 8 waves per SIMD
+  const int64_t J = type == DAS_SAFT ? (Lt + kSB - 1) / kSB
+                                     : ((Lr + kRC - 1) / kRC) * ((Lt + kTB - 1) / kTB);
+  FwdShape sh{4, 1};
+  if (tiles * 4 < target) sh.waves = kMaxWaves;
+  sh.waves = (int) std::max<int64_t>(1, std::min<int64_t>(sh.waves, J));
+  const int64_t waves_total = tiles * sh.waves;
+  if (waves_total * 2 <= target) {
+    const int64_t max_split = std::max<int64_t>(1, (J + sh.waves - 1) / sh.waves);
+    sh.split = (int) std::min<int64_t>(std::min<int64_t>((target + waves_total - 1) / waves_total,
+                                                         max_split), 64);
+  }
+  return sh;
+}
+
+template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
+void fwd_launch(const DasArgs &a, hipStream_t s)
+{
+  FwdShape sh = forward_shape(TYPE, a.No, a.n_frames, a.Lt, a.Lr);
+  if (sh.split > 1 && (size_t) sh.split * a.n_frames * a.No * sizeof(DT) > a.scratch_bytes)
+    sh.split = 1;
+  dim3 grid((unsigned) ((a.No + 63) / 64), (unsigned) sh.split, (unsigned) a.n_frames);
+  dim3 block((unsigned) (64 * sh.waves));
+  if (a.status)
+    hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, true>), grid, block, 0, s, a,
+                       a.perm, sh.split, (DT *) a.scratch);
+  else
+    hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, false>), grid, block, 0, s, a,
+                       a.perm, sh.split, (DT *) a.scratch);
+  if (sh.split > 1) {
+    const int64_t n = a.No * a.n_frames;
+    hipLaunchKernelGGL(das_reduce_kernel<DT>, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0,
+                       s, (const DT *) a.scratch, sh.split, a.No, a.n_frames, a.im_stride,
+                       (DT *) a.Im);
+  }
 }
 
 template <typename DT, typename GT, int TYPE>
-void fwd_dispatch_interp(const DasArgs &a, const int32_t *perm, int interp, bool apod,
-                         hipStream_t s)
+void fwd_dispatch_interp(const DasArgs &a, int interp, bool apod, hipStream_t s)
 {
   if (interp == DREAM_HIP_NEAREST) {
-    if (apod) fwd_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, true>(a, perm, s);
-    else fwd_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, false>(a, perm, s);
+    if (apod) fwd_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, true>(a, s);
+    else fwd_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, false>(a, s);
   } else {
-    if (apod) fwd_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, true>(a, perm, s);
-    else fwd_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, false>(a, perm, s);
+    if (apod) fwd_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, true>(a, s);
+    else fwd_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, false>(a, s);
   }
 }
 
 template <typename DT, typename GT>
-void fwd_dispatch_type(const DasArgs &a, const int32_t *perm, int type, int interp, bool apod,
-                       hipStream_t s)
+void fwd_dispatch_type(const DasArgs &a, int type, int interp, bool apod, hipStream_t s)
 {
   switch (type) {
-  case DAS_SAFT: fwd_dispatch_interp<DT, GT, DAS_SAFT>(a, perm, interp, apod, s); break;
-  case DAS_TFM: fwd_dispatch_interp<DT, GT, DAS_TFM>(a, perm, interp, apod, s); break;
-  case DAS_RCA_COLTX: fwd_dispatch_interp<DT, GT, DAS_RCA_COLTX>(a, perm, interp, apod, s); break;
-  default: fwd_dispatch_interp<DT, GT, DAS_RCA_ROWTX>(a, perm, interp, apod, s); break;
+  case DAS_SAFT: fwd_dispatch_interp<DT, GT, DAS_SAFT>(a, interp, apod, s); break;
+  case DAS_TFM: fwd_dispatch_interp<DT, GT, DAS_TFM>(a, interp, apod, s); break;
+  case DAS_RCA_COLTX: fwd_dispatch_interp<DT, GT, DAS_RCA_COLTX>(a, interp, apod, s); break;
+  default: fwd_dispatch_interp<DT, GT, DAS_RCA_ROWTX>(a, interp, apod, s); break;
   }
 }
 
@@ -626,20 +808,67 @@ size_t geom_size(int prec) { return prec == DREAM_HIP_F32_REF ? 4 : 8; }
 
 } // namespace
 
+size_t das_forward_scratch_bytes(int type, int prec, int64_t No, int64_t n_frames, int64_t Lt,
+                                 int64_t Lr)
+{
+  const FwdShape sh = forward_shape(Lr == 0 ? DAS_SAFT : type, No, n_frames, Lt, Lr);
+  return sh.split > 1 ? (size_t) sh.split * (size_t) n_frames * (size_t) No * elem_size(prec) : 0;
+}
+
+template <typename GT>
+void pixel_order(const GT *Ro, int64_t No, int32_t *perm)
+{
+  double lo[3], hi[3];
+  for (int c = 0; c < 3; c++) {
+    lo[c] = hi[c] = No ? (double) Ro[c * No] : 0.0;
+    for (int64_t n = 1; n < No; n++) {
+      const double v = (double) Ro[n + c * No];
+      lo[c] = std::min(lo[c], v);
+      hi[c] = std::max(hi[c], v);
+    }
+  }
+  // Spread 10-bit cell indices to every third bit and interleave (z, y, x).
+  auto spread = [](uint32_t v) {
+    uint64_t x = v & 0x3ff;
+    x = (x | (x << 16)) & 0x030000FFull;
+    x = (x | (x << 8)) & 0x0300F00Full;
+    x = (x | (x << 4)) & 0x030C30C3ull;
+    x = (x | (x << 2)) & 0x09249249ull;
+    return x;
+  };
+  std::vector<std::pair<uint64_t, int32_t>> key((size_t) No);
+  for (int64_t n = 0; n < No; n++) {
+    uint64_t code = 0;
+    for (int c = 0; c < 3; c++) {
+      const double span = hi[c] - lo[c];
+      const uint32_t q = span > 0 ? (uint32_t) std::min(1023.0, ((double) Ro[n + c * No] - lo[c]) / span * 1023.0) : 0u;
+      code |= spread(q) << c;
+    }
+    key[(size_t) n] = {code, (int32_t) n};
+  }
+  std::stable_sort(key.begin(), key.end(),
+                   [](const std::pair<uint64_t, int32_t> &x, const std::pair<uint64_t, int32_t> &y) {
+                     return x.first < y.first;
+                   });
+  for (int64_t n = 0; n < No; n++) perm[n] = key[(size_t) n].second;
+}
+
+template void pixel_order<double>(const double *, int64_t, int32_t *);
+template void pixel_order<float>(const float *, int64_t, int32_t *);
+
 void launch_das_forward(const DasArgs &a, int type, int prec, int interp, bool apod,
                         hipStream_t s)
 {
-  const int32_t *perm = nullptr;
-  if (prec == DREAM_HIP_F64) fwd_dispatch_type<double, double>(a, perm, type, interp, apod, s);
-  else if (prec == DREAM_HIP_F32) fwd_dispatch_type<float, double>(a, perm, type, interp, apod, s);
-  else fwd_dispatch_type<float, float>(a, perm, type, interp, apod, s);
+  if (prec == DREAM_HIP_F64) fwd_dispatch_type<double, double>(a, type, interp, apod, s);
+  else if (prec == DREAM_HIP_F32) fwd_dispatch_type<float, double>(a, type, interp, apod, s);
+  else fwd_dispatch_type<float, float>(a, type, interp, apod, s);
   DREAM_HIP_CHECK(hipGetLastError());
 }
 
 void launch_das_adjoint(const DasArgs &a, int type, int prec, int interp, bool apod,
                         bool want_gy, hipStream_t s)
 {
-  const int32_t *perm = nullptr;
+  const int32_t *perm = a.perm;
   if (prec == DREAM_HIP_F64) adj_dispatch_type<double, double>(a, perm, type, interp, apod, want_gy, s);
   else if (prec == DREAM_HIP_F32) adj_dispatch_type<float, double>(a, perm, type, interp, apod, want_gy, s);
   else throw Error(DREAM_HIP_EINVAL, "the adjoint is not available for DREAM_HIP_F32_REF handles");
@@ -655,7 +884,7 @@ DasHandle::DasHandle(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, in
   require(precision == DREAM_HIP_F64 || precision == DREAM_HIP_F32_REF ||
               precision == DREAM_HIP_F32,
           "das: unknown precision");
-  require(K >= 1 && K < (1ll << 31), "das: A-scan length out of range");
+  require(K >= 1 && K < (1ll << 26), "das: A-scan length out of range (1 .. 2^26)");
   require(No >= 0 && Lt >= 1 && Lr >= 0, "das: bad array sizes");
   const int64_t n_ascans = type_ == DAS_SAFT ? Lt : Lt * Lr;
   require(n_ascans * K < (1ll << 40), "das: data too large");
@@ -670,6 +899,37 @@ DasHandle::~DasHandle()
   (void) hipHostFree(h_status_);
   (void) hipFree(d_buf_);
   (void) hipFree(d_z_);
+  (void) hipFree(d_scratch_);
+  (void) hipFree(d_perm_);
+}
+
+void DasHandle::ensure_scratch(size_t bytes)
+{
+  if (bytes <= d_scratch_bytes_) return;
+  (void) hipFree(d_scratch_);
+  d_scratch_ = nullptr;
+  d_scratch_bytes_ = 0;
+  if (hipMalloc(&d_scratch_, bytes) != hipSuccess)
+    throw Error(DREAM_HIP_ENOMEM, "das: device allocation failed (partial sums)");
+  d_scratch_bytes_ = bytes;
+}
+
+void DasHandle::set_pixel_order(const int32_t *perm, int64_t n)
+{
+  if (!perm) {
+    (void) hipFree(d_perm_);
+    d_perm_ = nullptr;
+    return;
+  }
+  require(n == No_, "das: the pixel order must have one entry per pixel");
+  std::vector<char> seen((size_t) No_, 0);
+  for (int64_t q = 0; q < n; q++) {
+    require(perm[q] >= 0 && perm[q] < No_ && !seen[(size_t) perm[q]],
+            "das: the pixel order must be a permutation of 0..No-1");
+    seen[(size_t) perm[q]] = 1;
+  }
+  if (!d_perm_) DREAM_HIP_CHECK(hipMalloc(&d_perm_, (size_t) std::max<int64_t>(No_, 1) * 4));
+  if (n) DREAM_HIP_CHECK(hipMemcpy(d_perm_, perm, (size_t) n * 4, hipMemcpyHostToDevice));
 }
 
 bool DasHandle::changed(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr,
@@ -707,6 +967,7 @@ void DasHandle::fill_args(DasArgs &a, const void *Y, const void *Ro, const void 
     a.inv_cp = 1.0 / cp;
     a.cp = cp;
   }
+  a.leg_max = std::ldexp(1.0, 26) / a.fs_khz;
   a.apod_t = apod_t;
   a.apod_r = apod_r;
   a.n_frames = n_frames;
@@ -714,6 +975,9 @@ void DasHandle::fill_args(DasArgs &a, const void *Y, const void *Ro, const void 
   a.im_stride = im_stride;
   a.Im = Im;
   a.status = d_status_;
+  a.perm = d_perm_;
+  a.scratch = d_scratch_;
+  a.scratch_bytes = d_scratch_bytes_;
 }
 
 int DasHandle::run_dev(const void *Y, const void *Ro, const void *Gt, const void *Gr, double dt,
@@ -733,24 +997,14 @@ int DasHandle::run_dev(const void *Y, const void *Ro, const void *Gt, const void
   const bool apod = apod_t || apod_r;
   require(!apod || apod_t, "das: apod_r given without apod_t");
   require(!apod || type_ == DAS_SAFT || apod_r, "das: TFM/RCA apodization needs apod_r");
+  require((uint64_t) y_stride * elem_size(prec_) < 0xFFFFFFF0ull,
+          "das: one frame of Y must be smaller than 4 GiB");
+  ensure_scratch(das_forward_scratch_bytes(type_, prec_, No_, n_frames, Lt_, Lr_));
   DasArgs a;
   fill_args(a, Y, Ro, Gt, Gr, dt, delay_type, delay, cp, apod_t, apod_r, n_frames, y_stride,
             im_stride, Im);
-  if (!a.delay_multi && No_ > 0) {
-    // The scalar delay is read here once so its ms value is exact host math.
-    double d;
-    if (geom_size(prec_) == 8) {
-      DREAM_HIP_CHECK(hipMemcpyAsync(&d, delay, 8, hipMemcpyDeviceToHost, s));
-      DREAM_HIP_CHECK(hipStreamSynchronize(s));
-    } else {
-      float f;
-      DREAM_HIP_CHECK(hipMemcpyAsync(&f, delay, 4, hipMemcpyDeviceToHost, s));
-      DREAM_HIP_CHECK(hipStreamSynchronize(s));
-      d = f;
-    }
-    a.delay_ms = prec_ == DREAM_HIP_F32_REF ? (double) (float) (d / 1000.0) : d / 1000.0;
-  }
-  zero_async(d_status_, sizeof(Status), s);
+  if (want_status) zero_async(d_status_, sizeof(Status), s);
+  else a.status = nullptr;
   if (No_ > 0) launch_das_forward(a, type_, prec_, interp, apod, s);
   if (!want_status) return E_NONE;
   DREAM_HIP_CHECK(hipMemcpyAsync(h_status_, d_status_, sizeof(Status), hipMemcpyDeviceToHost, s));
@@ -799,6 +1053,13 @@ int DasHandle::run_host(const void *Y, const void *Ro, const void *Gt, const voi
       DREAM_HIP_CHECK(hipMemcpy(dst, w.data(), (size_t) n * 8, hipMemcpyHostToDevice));
     }
   };
+  // Visit the pixels in a spatially compact order.
+  {
+    std::vector<int32_t> perm((size_t) No_);
+    if (prec_ == DREAM_HIP_F64) pixel_order((const double *) Ro, No_, perm.data());
+    else pixel_order((const float *) Ro, No_, perm.data());
+    set_pixel_order(perm.data(), No_);
+  }
   put_geom(dRo, Ro, No_ * 3);
   put_geom(dGt, Gt, Lt_ * 3);
   if (bGr) put_geom(dGr, Gr, Lr_ * 3);

@@ -15,13 +15,28 @@ struct DasArgs {
   const void *Gr;  int32_t Lr;
   const void *delay;  int32_t delay_multi;
   double fs_khz, inv_cp, delay_ms, cp;   // in the geometry type, passed as double
+  double leg_max;                        // legs are clamped to +-leg_max (2^29 / fs)
   const void *apod_t, *apod_r;
   int64_t n_frames, y_stride, im_stride;
   void *Im;
   Status *status;
   // Adjoint-only.
   const void *g;  void *gY;  void *z_t;  void *z_r;  void *z_cp;
+  // Pixel visiting order (device, No entries) or null for 0..No-1, and the
+  // handle's scratch for partial sums of split forward launches.
+  const int32_t *perm;
+  void *scratch;  size_t scratch_bytes;
 };
+
+// Bytes of forward partial-sum scratch a launch of this shape uses.
+size_t das_forward_scratch_bytes(int type, int prec, int64_t No, int64_t n_frames, int64_t Lt,
+                                 int64_t Lr);
+
+// Spatially compact visiting order of the pixels (Morton order of the
+// quantised coordinates), so the 64 pixels of a wave form a compact tile and
+// their gathers share cache lines.  Ro is No x 3 column-major.
+template <typename GT>
+void pixel_order(const GT *Ro, int64_t No, int32_t *perm);
 
 class DasHandle {
 public:
@@ -63,6 +78,15 @@ private:
   size_t d_buf_bytes_ = 0;
   void *d_z_ = nullptr;          // adjoint channel accumulators
   size_t d_z_bytes_ = 0;
+  void *d_scratch_ = nullptr;    // forward partial sums (split launches)
+  size_t d_scratch_bytes_ = 0;
+  int32_t *d_perm_ = nullptr;    // pixel order (No entries) or null
+public:
+  // Set (perm != null, n == No) or clear (perm == null) the pixel order; perm
+  // is a host array, copied.
+  void set_pixel_order(const int32_t *perm, int64_t n);
+private:
+  void ensure_scratch(size_t bytes);
 };
 
 void launch_das_forward(const DasArgs &a, int type, int prec, int interp, bool apod,

@@ -36,6 +36,16 @@ def _dev_tensor(x, dtype, device):
     return t.contiguous()
 
 
+def pixel_order_of(Ro):
+    """Spatially compact visiting order of the points Ro ((No, 3))."""
+    R = np.asfortranarray(np.asarray(Ro.cpu() if torch.is_tensor(Ro) else Ro,
+                                     dtype=np.float64).reshape(-1, 3))
+    perm = np.empty(R.shape[0], dtype=np.int32)
+    _lib.check(_lib.lib().dream_hip_pixel_order(
+        R.ctypes.data, R.shape[0], perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    return perm
+
+
 class DasPlan:
     """A DAS object on the GPU (DAS<T> with cached state) for fixed geometry.
 
@@ -43,7 +53,8 @@ class DasPlan:
     row-per-element convention; stored on the device coordinate-major.
     """
 
-    def __init__(self, method, K, Ro, Gt, Gr=None, precision="f32", device="cuda"):
+    def __init__(self, method, K, Ro, Gt, Gr=None, precision="f32", device="cuda",
+                 pixel_order=True):
         _lib.require_device()
         self.method = method
         self.type = _lib.DAS_TYPE[method]
@@ -70,6 +81,17 @@ class DasPlan:
         _lib.check(_lib.lib().dream_hip_das_create(self.type, self.K, self.No, self.Lt, self.Lr,
                                                    self.prec, ctypes.byref(self.h)))
         self._delay_cache = {}
+        if pixel_order:
+            self.set_pixel_order(pixel_order_of(Ro))
+
+    def set_pixel_order(self, perm):
+        """Visit the pixels in `perm` order (None: 0..No-1)."""
+        if perm is None:
+            _lib.check(_lib.lib().dream_hip_das_set_pixel_order(self.h, None, 0))
+            return
+        perm = np.ascontiguousarray(perm, dtype=np.int32)
+        _lib.check(_lib.lib().dream_hip_das_set_pixel_order(
+            self.h, perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), perm.size))
 
     def __del__(self):
         try:

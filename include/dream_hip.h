@@ -177,6 +177,18 @@ int dream_hip_das_create(int das_type, int64_t a_scan_len, int64_t No,
                          void **handle);
 int dream_hip_das_destroy(void *handle);
 
+/* Order in which the handle visits the pixels (a permutation of 0..No-1,
+ * HOST array, copied; NULL restores 0..No-1).  The result is the same for
+ * any order up to the summation order; a spatially compact order makes each
+ * wave's 64 pixels a compact tile whose gathers share cache lines.
+ * dream_hip_das orders the pixels itself with dream_hip_pixel_order. */
+int dream_hip_das_set_pixel_order(void *handle, const int32_t *perm, int64_t n);
+
+/* Spatially compact order of No points (Morton order of the coordinates
+ * quantised to 1024 cells per axis over their bounding box).  Ro is No x 3
+ * column-major FP64 (HOST); perm (HOST, No entries) receives the order. */
+int dream_hip_pixel_order(const double *Ro, int64_t No, int32_t *perm);
+
 /* 1 if (das_type, a_scan_len, No, num_t, num_r, precision) differ from the
  * handle's (DAS<T>::das_setup_has_changed, include/das.h:77-94). */
 int dream_hip_das_setup_has_changed(void *handle, int das_type, int64_t a_scan_len,

@@ -197,3 +197,23 @@ def test_autograd_function(gpu, oracle):
     assert rel_l2(Yt.grad.cpu().numpy().T, gY) <= 1e-10
     assert rel_l2(at.grad.cpu().numpy(), gat) <= 1e-10
     assert abs(float(cp.grad) - gcp) <= 1e-9 * abs(gcp)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_split_launch_small_image(gpu, oracle, prec):
+    """Few pixels and many channel blocks: the forward launch splits tiles
+    over work-groups and adds partial sums in a second kernel."""
+    from dream_amd.torch_ops import DasPlan
+    rng = np.random.default_rng(8)
+    L, K, No = 64, 900, 256
+    x = np.linspace(-6.0, 6.0, L)
+    G = np.c_[x, np.zeros(L), np.zeros(L)]
+    Y = rng.standard_normal((K, L * L))
+    Ro = np.c_[rng.uniform(-6, 6, No), np.zeros(No), rng.uniform(1, 8, No)]
+    ref, _ = oracle.das(Y, G, G, Ro, 0.02, 0.05, 1500.0, "tfm", "ignore", "f64")
+    dt_ = torch.float64 if prec == "f64" else torch.float32
+    plan = DasPlan("tfm", K, Ro, G, G, precision=prec)
+    Yt = torch.tensor(Y.T.copy(), dtype=dt_, device="cuda")
+    for _ in range(2):
+        Im = plan.forward(Yt, 0.02, 0.05, 1500.0).double().cpu().numpy()
+        assert rel_l2(Im, ref) <= (1e-13 if prec == "f64" else 2e-6)
