@@ -94,6 +94,114 @@ def bench_c2(steps, warmup, want_cpu, want_extra=True):
     return res, extra, cpu
 
 
+def _sir_device_problem(p, torch):
+    q = dict(p)
+    q["Ro"] = torch.tensor(p["Ro"].T.copy(), device="cuda")
+    d = p["delay"]
+    q["delay"] = torch.tensor(d, device="cuda") if hasattr(d, "__len__") else float(d)
+    return q
+
+
+def _sir_contributions(p, oracle_free=True):
+    """Sub-element contributions of one SIR problem (points x elements x subs)."""
+    import numpy as np
+    nx = len(_subgrid(p["a"], p["dx"]))
+    ny = len(_subgrid(p["b"], p["dy"]))
+    return p["Ro"].shape[0] * np.asarray(p["G"]).reshape(-1, 3).shape[0] * nx * ny
+
+
+def _subgrid(w, d):
+    out, s = [], -w / 2.0 + d / 2.0
+    while s <= w / 2.0:
+        out.append(s)
+        s += d
+    return out
+
+
+def bench_sir(steps, warmup):
+    """C1 (FP64, one problem) and C3 (FP32, 32 uneven problems, one persistent launch
+    under a captured graph, against one dream_arr_rect_dev call per problem)."""
+    import numpy as np
+    import torch
+
+    import workloads
+    from dream_amd.torch_ops import SirBatch, arr_rect
+    out = {}
+    # C1
+    w = workloads.c1()
+    batch = SirBatch([_sir_device_problem(w, torch)], out_dtype=torch.float64)
+    ms = time_launches(lambda: batch.run(use_graph=True), steps, warmup)
+    n_oe = w["Ro"].shape[0] * w["G"].shape[0]
+    contrib = _sir_contributions(w)
+    out["c1_sir_f64"] = dict(ms=round(ms, 4), obs_elem_per_s=n_oe / (ms * 1e-3),
+                             contributions_per_s=contrib / (ms * 1e-3),
+                             roofline_frac=(contrib * 16 + w["Ro"].shape[0] * w["nt"] * 8)
+                             / (ms * 1e-3) / HBM_PEAK)
+    # C3
+    probs = workloads.c3()
+    dev = [_sir_device_problem(p, torch) for p in probs]
+    n_oe = sum(p["Ro"].shape[0] * p["G"].shape[0] for p in probs)
+    contrib = sum(_sir_contributions(p) for p in probs)
+    out_bytes = sum(p["Ro"].shape[0] * p["nt"] * 4 for p in probs)
+    log(f"C3: {len(probs)} problems, {n_oe} obs-elem, {contrib:.3e} contributions")
+    batch = SirBatch(dev, out_dtype=torch.float32)
+    ms_b = time_launches(lambda: batch.run(use_graph=True), max(2, steps // 10), 1)
+    codes, oobs = batch.status()
+    # In-bounds contributions exactly: total minus the kernel's out-of-bounds count.
+    inb = contrib - sum(oobs)
+    Hs = [torch.empty_like(H) for H in batch.H]
+
+    def per_call():
+        for p, q, H in zip(probs, dev, Hs):
+            arr_rect(q["Ro"], p["a"], p["b"], p["dx"], p["dy"], p["dt"], p["nt"], q["delay"],
+                     p["v"], p["cp"], p["G"], foc_met="ud", focal=p["focal"], apod_met="hann",
+                     err_level="ignore", out_dtype=torch.float32, out=H)
+    ms_p = time_launches(per_call, max(1, steps // 25), 1)
+    out["c3_sir_batched_f32"] = dict(
+        ms=round(ms_b, 3), obs_elem_per_s=n_oe / (ms_b * 1e-3),
+        contributions_per_s=contrib / (ms_b * 1e-3), in_bounds_contributions=int(inb),
+        roofline_frac=(inb * 8 + out_bytes) / (ms_b * 1e-3) / HBM_PEAK,
+        per_call_ms=round(ms_p, 3), speedup_vs_per_call=ms_p / ms_b)
+    return out
+
+
+def bench_das_diff(steps, warmup):
+    """C4 SAFT (linear, apodized) forward and adjoint; C5 RCA 8 frames forward and
+    apodization adjoint."""
+    import torch
+
+    import workloads
+    from dream_amd.torch_ops import DasPlan
+    out = {}
+    w = workloads.c4(device="cuda", dtype=torch.float32)
+    plan = DasPlan("saft", w["K"], w["Ro"], w["Gt"], None, precision="f32")
+    Y = w["Y"].contiguous()
+    apod = torch.ones(plan.Lt, dtype=torch.float32, device="cuda")
+    g = torch.randn(plan.No, dtype=torch.float32, device="cuda")
+    fwd = time_launches(lambda: plan.forward(Y, w["dt"], 0.0, w["cp"], apod, None, "linear"),
+                        steps, warmup)
+    bwd = time_launches(lambda: plan.adjoint(Y, g, w["dt"], 0.0, w["cp"], apod, None, "linear"),
+                        steps, warmup)
+    pairs = plan.No * plan.n_ascans
+    out["c4_saft_linear_f32"] = dict(fwd_ms=round(fwd, 4), bwd_ms=round(bwd, 4),
+                                     bwd_over_fwd=bwd / fwd, sums_per_s=pairs / (fwd * 1e-3))
+    del plan, Y
+    torch.cuda.empty_cache()
+    w = workloads.c5(device="cuda", dtype=torch.float32)
+    plan = DasPlan("rca_rowtx", w["K"], w["Ro"], w["Gt"], w["Gr"], precision="f32")
+    Y = w["Y"].contiguous()
+    F = Y.shape[0]
+    ones = torch.ones(plan.Lt, dtype=torch.float32, device="cuda")
+    g = torch.randn((F, plan.No), dtype=torch.float32, device="cuda")
+    fwd = time_launches(lambda: plan.forward(Y, w["dt"], 0.0, w["cp"]), 2, 1)
+    bwd = time_launches(lambda: plan.adjoint(Y, g, w["dt"], 0.0, w["cp"], ones, ones, "nearest",
+                                             want_gy=False, want_cp=False), 2, 1)
+    pairs = plan.No * plan.n_ascans * F
+    out["c5_rca_8frames_f32"] = dict(fwd_ms=round(fwd, 3), bwd_apod_ms=round(bwd, 3),
+                                     bwd_over_fwd=bwd / fwd, sums_per_s=pairs / (fwd * 1e-3))
+    return out
+
+
 def cpu_baseline_c2(w, n_pix, n_ch):
     import numpy as np
 
@@ -147,6 +255,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("calibration copy: 3 x 268435456 bytes")
+    if not args.no_extra:
+        extra.update(bench_sir(args.steps, args.warmup))
+        extra.update(bench_das_diff(args.steps, args.warmup))
     line = {
         "metric": METRIC,
         "value": res["value"],

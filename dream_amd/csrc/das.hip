@@ -229,16 +229,27 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
     const int J = (Lt + kSB - 1) / kSB;
     for (int job = sidx * nwaves + wave; job < J; job += stride) {
       const int n0 = job * kSB;
+      GT u[kSB];
+      bool any_in = false;
+#pragma unroll
+      for (int q = 0; q < kSB; q++) {
+        const int n = min(n0 + q, Lt - 1);
+        GT t = leg3<GT>(Gt[n] - xo, Gt[n + Lt] - yo, Gt[n + 2 * Lt] - zo) * inv_cp;
+        t += dl;
+        u[q] = clamp_leg<GT>(t + t, tmax) * fs;    // (T)(2.0*t) * fs, 2t exact
+        any_in |= (n0 + q < Lt) && u[q] > GT(-1) && u[q] < (GT) K;
+      }
+      if (!__any(any_in && valid)) {               // the whole block is out of range
+        if (STATUS) oob = true;
+        continue;
+      }
       DT v[kSB];
 #pragma unroll
       for (int q = 0; q < kSB; q++) {
         const int n = min(n0 + q, Lt - 1);
         const bool live = n0 + q < Lt;
-        GT t = leg3<GT>(Gt[n] - xo, Gt[n + Lt] - yo, Gt[n + 2 * Lt] - zo) * inv_cp;
-        t += dl;
-        const GT tdp = clamp_leg<GT>(t + t, tmax);  // (T)(2.0*t), exact
         int k;
-        v[q] = fetch_ascan<DT, GT, INTERP>(Yf + (int64_t) n * K, live ? kbytes : 0u, tdp * fs, k);
+        v[q] = fetch_ascan<DT, GT, INTERP>(Yf + (int64_t) n * K, live ? kbytes : 0u, u[q], k);
         if (STATUS) oob |= live && (unsigned) k >= Ku;
       }
       DT part = DT(0);
@@ -268,6 +279,15 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
           tr[jj] = leg3<GT>(gx_r, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
         tr[jj] = clamp_leg<GT>(tr[jj], tmax);
       }
+      // Extremes of the chunk's receive legs: a transmit row whose sample
+      // positions (tt + tr) * fs all lie outside (-1, K) for every lane of
+      // the wave issues no gathers (rounding is monotone, so the bound is exact).
+      GT trmin = tr[0], trmax = tr[0];
+#pragma unroll
+      for (int jj = 1; jj < kRC; jj++) {
+        trmin = fmin(trmin, tr[jj]);       // clamped duplicates of the last
+        trmax = fmax(trmax, tr[jj]);       // receiver do not widen the range
+      }
       for (int i = i0; i < i1; i++) {
         GT tt;
         if (TYPE == DAS_TFM)
@@ -276,6 +296,11 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
           tt = leg3<GT>(Gt[i] - xo, gy_c, Gt[i + 2 * Lt] - zo) * inv_cp;
         tt += dl;
         tt = clamp_leg<GT>(tt, tmax);
+        const bool row_out = !valid || !((tt + trmin) * fs < (GT) K) || !((tt + trmax) * fs > GT(-1));
+        if (__all(row_out)) {
+          if (STATUS) oob = true;
+          continue;
+        }
         const DT wi = APOD ? at[i] : DT(1);
         // All kRC gathers of the block are issued before any is consumed.
         DT v[kRC];

@@ -64,7 +64,7 @@ def _echoes(tau_us, amp, K, fs_mhz, noise, gen, device, dtype):
         idx = torch.where(ok, k, torch.zeros_like(k)).long()
         val = torch.where(ok, val, torch.zeros_like(val))
         Y[a0:a0 + t.shape[0]].scatter_add_(1, idx.reshape(t.shape[0], -1), val.reshape(t.shape[0], -1))
-    Y += noise * torch.randn(Y.shape, generator=gen, dtype=torch.float64, device="cpu").to(device)
+    Y += noise * torch.randn(Y.shape, generator=gen, dtype=torch.float64, device=gen.device)
     return Y.to(dtype)
 
 
@@ -89,7 +89,7 @@ def c1():
 
 def c2(seed=2, device="cpu", dtype=torch.float32):
     rng = np.random.default_rng(seed)
-    gen = torch.Generator().manual_seed(seed)
+    gen = torch.Generator(device=device).manual_seed(seed)
     L, pitch, K, fs, cp = 64, 0.5, 2048, 50.0, 1500.0
     G = linear_array(L, pitch)
     S, amp = _scatterers(rng, 20, (-16.0, 16.0), (10.0, 40.0))
@@ -130,7 +130,7 @@ def c3(seed=3, n_problems=32):
 
 def c4(seed=4, device="cpu", dtype=torch.float32):
     rng = np.random.default_rng(seed)
-    gen = torch.Generator().manual_seed(seed)
+    gen = torch.Generator(device=device).manual_seed(seed)
     L, pitch, K, fs, cp = 128, 0.3, 4096, 50.0, 1500.0
     G = linear_array(L, pitch)
     S, amp = _scatterers(rng, 40, (-16.0, 16.0), (10.0, 40.0))
@@ -151,7 +151,7 @@ def rca_arrays(L=128, pitch=0.2):
 
 def c5(seed=5, device="cpu", dtype=torch.float32, n_frames=8, L=128, K=4096, grid=(128, 128, 256)):
     rng = np.random.default_rng(seed)
-    gen = torch.Generator().manual_seed(seed)
+    gen = torch.Generator(device=device).manual_seed(seed)
     pitch, fs, cp = 0.2, 40.0, 1540.0
     G_col, G_row = rca_arrays(L, pitch)
     ext = L * pitch / 2
