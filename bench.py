@@ -169,6 +169,16 @@ def bench_das_diff(steps, warmup):
     """C4 SAFT (linear, apodized) forward and adjoint; C5 RCA 8 frames forward and
     apodization adjoint."""
     import torch
+    out = bench_das_diff_c4(steps, warmup)
+    torch.cuda.empty_cache()
+    out.update(bench_das_diff_c5(steps, warmup))
+    return out
+
+
+def bench_das_diff_c4(steps, warmup):
+    """C4: SAFT, 128 A-scans x 4096 samples, 512 x 512 pixels, linear interpolation with
+    apodization; forward and the adjoint (gradients for Y, apodization and cp)."""
+    import torch
 
     import workloads
     from dream_amd.torch_ops import DasPlan
@@ -185,8 +195,17 @@ def bench_das_diff(steps, warmup):
     pairs = plan.No * plan.n_ascans
     out["c4_saft_linear_f32"] = dict(fwd_ms=round(fwd, 4), bwd_ms=round(bwd, 4),
                                      bwd_over_fwd=bwd / fwd, sums_per_s=pairs / (fwd * 1e-3))
-    del plan, Y
-    torch.cuda.empty_cache()
+    return out
+
+
+def bench_das_diff_c5(steps, warmup):
+    """C5: RCA row transmit, 128 x 128 A-scans x 4096 samples, 128 x 128 x 256 voxels,
+    8 frames in one launch; forward and the apodization adjoint."""
+    import torch
+
+    import workloads
+    from dream_amd.torch_ops import DasPlan
+    out = {}
     w = workloads.c5(device="cuda", dtype=torch.float32)
     plan = DasPlan("rca_rowtx", w["K"], w["Ro"], w["Gt"], w["Gr"], precision="f32")
     Y = w["Y"].contiguous()
@@ -233,6 +252,9 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: headline kernel only, plus a 256 MiB "
                          "device-to-device copy used to calibrate the HBM byte counters")
+    ap.add_argument("--part", choices=("sir", "c4", "c5"),
+                    help="profile one of the other paths alone (short run, no JSON line): "
+                         "sir = C1 + C3 SIR, c4 = SAFT forward/adjoint, c5 = RCA frames")
     args = ap.parse_args()
     if args.profile:
         args.no_cpu = True
@@ -245,6 +267,10 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     from dream_amd import _lib
     _lib.require_device()
+    if args.part:
+        part = {"sir": bench_sir, "c4": bench_das_diff_c4, "c5": bench_das_diff_c5}[args.part]
+        log(json.dumps(part(args.steps, args.warmup)))
+        return
 
     res, extra, cpu = bench_c2(args.steps, args.warmup, not args.no_cpu, not args.no_extra)
     if args.profile:

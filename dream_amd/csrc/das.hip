@@ -150,30 +150,30 @@ __device__ __forceinline__ DT fetch(__amdgpu_buffer_rsrc_t r, unsigned abase, un
   }
 }
 
-// Forward gathers use one buffer descriptor per A-scan (base = the A-scan's
-// first sample, size = K samples): the hardware returns 0 for any sample
-// index outside [0, K), so a gather is a shift and a load, with no compare.
-// Legs are clamped (leg_max = 2^26 / fs) so |u| <= 2^27 and a negative
-// index times sizeof(DT) never wraps back into [0, K).
-template <typename DT>
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t ascan_rsrc(const DT *ascan, unsigned kbytes)
-{
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<DT *>(ascan), (short) 0, (int) kbytes,
-                                           0x00020000);
-}
-
+// Forward gathers go through one buffer descriptor per frame.  The lane's
+// byte offset is (a*K + k) * sizeof(DT) when the sample index k lies in
+// [0, K), and kNoSample (past the frame, read as 0 by the hardware range
+// check) otherwise: one compare and one select per gather, and the whole
+// launch shares one descriptor (per-A-scan descriptors would need four
+// scalar registers each and spill when a row keeps 16 loads in flight).
+// Legs are clamped (leg_max = 2^26 / fs) so |u| <= 2^27 and the integer
+// conversion is exact.
 template <typename DT, typename GT, int INTERP>
-__device__ __forceinline__ DT fetch_ascan(const DT *ascan, unsigned kbytes, GT u, int &k)
+__device__ __forceinline__ DT fetch_ascan(__amdgpu_buffer_rsrc_t r, unsigned abase, unsigned K,
+                                          GT u, int &k)
 {
-  const __amdgpu_buffer_rsrc_t r = ascan_rsrc(ascan, kbytes);
+  constexpr unsigned s = (unsigned) sizeof(DT);
   if (INTERP == DREAM_HIP_NEAREST) {
     k = (int) u;                                // trunc toward zero
-    return bload<DT>(r, (unsigned) k * (unsigned) sizeof(DT));
+    const unsigned off = (unsigned) k < K ? (abase + (unsigned) k) * s : kNoSample;
+    return bload<DT>(r, off);
   } else {
     const GT fk = floor(u);
     k = (int) fk;
-    const DT y0 = bload<DT>(r, (unsigned) k * (unsigned) sizeof(DT));
-    const DT y1 = bload<DT>(r, (unsigned) (k + 1) * (unsigned) sizeof(DT));
+    const unsigned o0 = (unsigned) k < K ? (abase + (unsigned) k) * s : kNoSample;
+    const unsigned o1 = (unsigned) (k + 1) < K ? (abase + (unsigned) (k + 1)) * s : kNoSample;
+    const DT y0 = bload<DT>(r, o0);
+    const DT y1 = bload<DT>(r, o1);
     const DT f = (DT) (u - fk);
     return y0 + f * (y1 - y0);
   }
@@ -214,8 +214,8 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
   const GT tmax = (GT) A.leg_max;
   const int K = (int) A.K;
   const unsigned Ku = (unsigned) K;
-  const unsigned kbytes = Ku * (unsigned) sizeof(DT);
   const DT *Yf = (const DT *) A.Y + frame * A.y_stride;
+  const __amdgpu_buffer_rsrc_t rY = frame_rsrc(Yf, A.y_stride * (int64_t) sizeof(DT));
   const GT *Gt = (const GT *) A.Gt;
   const GT *Gr = (const GT *) A.Gr;
   const DT *at = (const DT *) A.apod_t;
@@ -249,7 +249,8 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
         const int n = min(n0 + q, Lt - 1);
         const bool live = n0 + q < Lt;
         int k;
-        v[q] = fetch_ascan<DT, GT, INTERP>(Yf + (int64_t) n * K, live ? kbytes : 0u, u[q], k);
+        v[q] = live ? fetch_ascan<DT, GT, INTERP>(rY, (unsigned) n * Ku, Ku, u[q], k) : DT(0);
+        if (!live) k = 0;
         if (STATUS) oob |= live && (unsigned) k >= Ku;
       }
       DT part = DT(0);
@@ -310,7 +311,7 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
             const int j = j0 + jj;
             const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
             int k;
-            v[jj] = fetch_ascan<DT, GT, INTERP>(Yf + (int64_t) a * K, kbytes, (tt + tr[jj]) * fs, k);
+            v[jj] = fetch_ascan<DT, GT, INTERP>(rY, (unsigned) a * Ku, Ku, (tt + tr[jj]) * fs, k);
             if (STATUS) oob |= (unsigned) k >= Ku;
           }
         } else {
@@ -320,8 +321,9 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
             const bool live = j0 + jj < Lr;
             const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
             int k;
-            v[jj] = fetch_ascan<DT, GT, INTERP>(Yf + (int64_t) a * K, live ? kbytes : 0u,
-                                                (tt + tr[jj]) * fs, k);
+            v[jj] = live ? fetch_ascan<DT, GT, INTERP>(rY, (unsigned) a * Ku, Ku, (tt + tr[jj]) * fs, k)
+                         : DT(0);
+            if (!live) k = 0;
             if (STATUS) oob |= live && (unsigned) k >= Ku;
           }
         }
@@ -387,142 +389,319 @@ __device__ __forceinline__ void fetch_taps(__amdgpu_buffer_rsrc_t r, unsigned ab
   }
 }
 
-// Channel-major adjoint: one work-group per (pixel block, A-scan, frame).
-// Scatters the A-scan's Y-gradient into LDS, then adds the touched window
-// into gY; accumulates sum_p g_p*value (z_t[a]) and sum_p g_p*w*slope*du/dcp
-// (z_cp) for the A-scan.  Each lane handles kPU pixels per step so that
-// their 2 kPU gathers are in flight together.
-constexpr int kPU = 4;
+// ------------------------------------------------------------ region adjoint
+
+// Region-major adjoint: one work-group per (region of kRegion consecutive
+// pixels in visiting order, group of A-scans, frame).  Each lane keeps its
+// kRPU pixels (coordinates and g) in registers for the whole group, so the
+// pixel data are read once per group rather than once per A-scan.  The
+// Y-gradient of every A-scan of the group is scattered into an LDS window of
+// kWin samples whose start comes from the distance range between the
+// element(s) and the region's bounding box (exact interval bound plus a
+// 2-sample margin); a contribution that falls outside its window (regions
+// with a large extent) is added to gY directly.  Windows are added into the
+// zeroed gY once at the end.  Per-A-scan sums of g*value (for the
+// apodization gradients) stay in registers and are reduced once.
+constexpr int kRPU = 4;                         // pixels per lane
+constexpr int kRegion = kThreads * kRPU;        // pixels per region
+constexpr int kSaftGroup = 8;                   // SAFT A-scans per work-group
+constexpr int kGI = 2, kGJ = 4;                 // two-leg group: transmits x receives
+constexpr int kWin = 512;                       // LDS window (samples) per A-scan
+
+template <typename T>
+__device__ __forceinline__ void lds_atomic_add(T *p, T v)
+{
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Range of |c - p| for p in [lo, hi] (c a point coordinate).
+template <typename GT>
+__device__ __forceinline__ void gap_range(GT c, GT lo, GT hi, GT &mn, GT &mx)
+{
+  const GT a = fabs(c - lo), b = fabs(c - hi);
+  mn = (c >= lo && c <= hi) ? GT(0) : fmin(a, b);
+  mx = fmax(a, b);
+}
+
+// Range of the stripe gap |stripe_gap(p, lo, hi)| for p in [pmin, pmax].
+template <typename GT>
+__device__ __forceinline__ void stripe_range(GT pmin, GT pmax, GT lo, GT hi, GT &mn, GT &mx)
+{
+  mn = (pmax < lo) ? lo - pmax : ((pmin > hi) ? pmin - hi : GT(0));
+  mx = fmax(fabs(stripe_gap<GT>(pmin, lo, hi)), fabs(stripe_gap<GT>(pmax, lo, hi)));
+}
+
+__device__ __forceinline__ int window_start(double umin)
+{
+  if (!(umin > -4.0e8)) umin = -4.0e8;          // also catches NaN
+  if (umin > 4.0e8) umin = 4.0e8;
+  return (int) floor(umin) - 2;
+}
 
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
 __global__ void __launch_bounds__(kThreads)
-das_adj_channel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t pix_per_block,
-                       int blocks_per_channel)
+das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj)
 {
+  constexpr int NA = TYPE == DAS_SAFT ? kSaftGroup : kGI * kGJ;
   extern __shared__ __align__(16) unsigned char smem[];
-  DT *gacc = reinterpret_cast<DT *>(smem);
-  __shared__ int s_kmin, s_kmax;
-  __shared__ double s_red[2][kThreads / 64];
-  const int K = (int) A.K;
-  const int64_t a = blockIdx.y;                 // A-scan
-  const int64_t frame = blockIdx.z;
-  const int blk = blockIdx.x;
-  const int Lt = A.Lt, Lr = A.Lr;
-  int i, j = 0;
-  if (TYPE == DAS_SAFT) { i = (int) a; }
-  else if (TYPE == DAS_RCA_ROWTX) { j = (int) (a / Lt); i = (int) (a - (int64_t) j * Lt); }
-  else { i = (int) (a / Lr); j = (int) (a - (int64_t) i * Lr); }
-  const bool want_gy = A.gY != nullptr;
+  // NA x kWin FP64 accumulators (only when gY is wanted): gfx950's LDS FP64
+  // atomic add is about 20 times faster than its FP32 one.
+  double *win = reinterpret_cast<double *>(smem);
+  __shared__ GT s_box[6][kThreads / 64];
+  __shared__ int s_w0[NA];
+  __shared__ double s_z[NA][kThreads / 64];
+  __shared__ double s_zc[kThreads / 64];
 
-  if (want_gy) for (int k = threadIdx.x; k < K; k += kThreads) gacc[k] = DT(0);
-  if (threadIdx.x == 0) { s_kmin = K; s_kmax = -1; }
-  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t region = blockIdx.x;
+  const int grp = blockIdx.y;
+  const int64_t frame = blockIdx.z;
+  const int Lt = A.Lt, Lr = A.Lr;
+  const int K = (int) A.K;
+  const bool want_gy = A.gY != nullptr;
+  // The group's A-scans: SAFT elements n0.. ; two-leg transmits i0.. x receives j0..
+  int n0 = 0, i0 = 0, j0 = 0;
+  if (TYPE == DAS_SAFT) n0 = grp * kSaftGroup;
+  else { i0 = (grp / n_gj) * kGI; j0 = (grp % n_gj) * kGJ; }
 
   const GT *Ro = (const GT *) A.Ro;
   const GT *Gt = (const GT *) A.Gt;
   const GT *Gr = (const GT *) A.Gr;
   const GT fs = (GT) A.fs_khz, inv_cp = (GT) A.inv_cp, dl = (GT) A.delay_ms;
-  const double dudc_scale = -(double) A.fs_khz / A.cp;    // du/dcp = -(travel time) fs / cp
   const GT tmax = (GT) A.leg_max;
-  const __amdgpu_buffer_rsrc_t ry =
-      frame_rsrc((const DT *) A.Y + frame * A.y_stride, A.y_stride * (int64_t) sizeof(DT));
-  const unsigned abase = (unsigned) (a * K);
   const DT *g = (const DT *) A.g + frame * A.im_stride;
   const DT *at = (const DT *) A.apod_t;
   const DT *ar = (const DT *) A.apod_r;
-  const DT w = APOD ? (TYPE == DAS_SAFT ? at[i] : at[i] * ar[j]) : DT(1);
   GT y_lo = 0, y_hi = 0, x_lo = 0, x_hi = 0;
   if (TYPE == DAS_RCA_COLTX || TYPE == DAS_RCA_ROWTX) {
     y_lo = Gr[Lr]; y_hi = Gr[2 * Lr - 1]; x_lo = Gt[0]; x_hi = Gt[Lt - 1];
   }
-  const int64_t p0 = (int64_t) blk * pix_per_block;
-  const int64_t p1 = min(p0 + pix_per_block, A.No);
-  double z = 0.0, zc = 0.0;
-  int kmin = K, kmax = -1;
-  for (int64_t base = p0 + threadIdx.x; base < p1; base += (int64_t) kThreads * kPU) {
-    GT u[kPU], tsum[kPU];
-    DT gp[kPU];
-    bool live[kPU];
+
+  // The lane's pixels.
+  GT xo[kRPU], yo[kRPU], zo[kRPU];
+  DT gp[kRPU];
+  bool live[kRPU];
+  const int64_t base = region * kRegion;
+  GT bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
-    for (int q = 0; q < kPU; q++) {
-      const int64_t idx = base + (int64_t) q * kThreads;
-      live[q] = idx < p1;
-      const int64_t p = pixel_of(A, perm, live[q] ? idx : p0);
-      const GT xo = Ro[p], yo = Ro[p + A.No], zo = Ro[p + 2 * A.No];
-      gp[q] = live[q] ? g[p] : DT(0);
+  for (int q = 0; q < kRPU; q++) {
+    const int64_t idx = base + (int64_t) q * kThreads + threadIdx.x;
+    live[q] = idx < A.No;
+    const int64_t p = pixel_of(A, perm, live[q] ? idx : base);
+    xo[q] = Ro[p]; yo[q] = Ro[p + A.No]; zo[q] = Ro[p + 2 * A.No];
+    gp[q] = live[q] ? g[p] : DT(0);
+    if (live[q]) {
+      bmin[0] = fmin(bmin[0], xo[q]); bmax[0] = fmax(bmax[0], xo[q]);
+      bmin[1] = fmin(bmin[1], yo[q]); bmax[1] = fmax(bmax[1], yo[q]);
+      bmin[2] = fmin(bmin[2], zo[q]); bmax[2] = fmax(bmax[2], zo[q]);
+    }
+  }
+  // Bounding box of the region (only needed for the LDS windows).
+  if (want_gy) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      GT mn = bmin[c], mx = bmax[c];
+      for (int off = 32; off > 0; off >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, off, 64));
+        mx = fmax(mx, __shfl_xor(mx, off, 64));
+      }
+      if (lane == 0) { s_box[c][wave] = mn; s_box[3 + c][wave] = mx; }
+    }
+    for (int s = threadIdx.x; s < NA * kWin; s += kThreads) win[s] = 0.0;
+    __syncthreads();
+    if (threadIdx.x < NA) {
+      GT lo[3], hi[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        lo[c] = s_box[c][0]; hi[c] = s_box[3 + c][0];
+        for (int w = 1; w < kThreads / 64; w++) {
+          lo[c] = fmin(lo[c], s_box[c][w]); hi[c] = fmax(hi[c], s_box[3 + c][w]);
+        }
+      }
+      // Distance ranges of the legs to the box, then the sample range.
+      double umin;
       if (TYPE == DAS_SAFT) {
-        GT t = leg3<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
+        const int n = min(n0 + (int) threadIdx.x, Lt - 1);
+        GT m0, m1, m2, x0, x1, x2;
+        gap_range<GT>(Gt[n], lo[0], hi[0], m0, x0);
+        gap_range<GT>(Gt[n + Lt], lo[1], hi[1], m1, x1);
+        gap_range<GT>(Gt[n + 2 * Lt], lo[2], hi[2], m2, x2);
+        const double dmin = sqrt((double) (m0 * m0 + m1 * m1 + m2 * m2));
+        umin = 2.0 * (dmin * (double) inv_cp + (double) dl) * (double) fs;
+      } else {
+        const int i = min(i0 + (int) threadIdx.x / kGJ, Lt - 1);
+        const int j = min(j0 + (int) threadIdx.x % kGJ, Lr - 1);
+        GT m0, m1, m2, x0, x1, x2;
+        if (TYPE == DAS_TFM) {
+          gap_range<GT>(Gt[i], lo[0], hi[0], m0, x0);
+          gap_range<GT>(Gt[i + Lt], lo[1], hi[1], m1, x1);
+        } else {
+          gap_range<GT>(Gt[i], lo[0], hi[0], m0, x0);
+          stripe_range<GT>(lo[1], hi[1], y_lo, y_hi, m1, x1);
+        }
+        gap_range<GT>(Gt[i + 2 * Lt], lo[2], hi[2], m2, x2);
+        const double dt_min = sqrt((double) (m0 * m0 + m1 * m1 + m2 * m2));
+        if (TYPE == DAS_TFM) {
+          gap_range<GT>(Gr[j], lo[0], hi[0], m0, x0);
+        } else {
+          stripe_range<GT>(lo[0], hi[0], x_lo, x_hi, m0, x0);
+        }
+        gap_range<GT>(Gr[j + Lr], lo[1], hi[1], m1, x1);
+        gap_range<GT>(Gr[j + 2 * Lr], lo[2], hi[2], m2, x2);
+        const double dr_min = sqrt((double) (m0 * m0 + m1 * m1 + m2 * m2));
+        umin = ((dt_min * (double) inv_cp + (double) dl) + dr_min * (double) inv_cp) * (double) fs;
+      }
+      s_w0[threadIdx.x] = window_start(umin);
+    }
+  }
+  __syncthreads();
+
+  // Two-leg: receive legs of the group for the lane's pixels.
+  GT tr[TYPE == DAS_SAFT ? 1 : kGJ][kRPU];
+  GT gx_r[kRPU], gy_c[kRPU];
+  if (TYPE != DAS_SAFT) {
+#pragma unroll
+    for (int q = 0; q < kRPU; q++) {
+      gx_r[q] = GT(0); gy_c[q] = GT(0);
+      if (TYPE != DAS_TFM) {
+        gy_c[q] = stripe_gap<GT>(yo[q], y_lo, y_hi);
+        gx_r[q] = stripe_gap<GT>(xo[q], x_lo, x_hi);
+      }
+#pragma unroll
+      for (int jj = 0; jj < kGJ; jj++) {
+        const int j = min(j0 + jj, Lr - 1);
+        if (TYPE == DAS_TFM)
+          tr[jj][q] = leg3<GT>(Gr[j] - xo[q], Gr[j + Lr] - yo[q], Gr[j + 2 * Lr] - zo[q]) * inv_cp;
+        else
+          tr[jj][q] = leg3<GT>(gx_r[q], Gr[j + Lr] - yo[q], Gr[j + 2 * Lr] - zo[q]) * inv_cp;
+      }
+    }
+  }
+
+  const __amdgpu_buffer_rsrc_t ry =
+      frame_rsrc((const DT *) A.Y + frame * A.y_stride, A.y_stride * (int64_t) sizeof(DT));
+  DT *gy_frame = want_gy ? (DT *) A.gY + frame * A.y_stride : nullptr;
+  DT zsum[NA];
+  DT zc = DT(0);
+#pragma unroll
+  for (int s = 0; s < NA; s++) zsum[s] = DT(0);
+
+#pragma unroll
+  for (int s = 0; s < NA; s++) {
+    int a, ii = 0, jj = 0;
+    bool ok;
+    DT w;
+    if (TYPE == DAS_SAFT) {
+      const int n = n0 + s;
+      ok = n < Lt;
+      a = min(n, Lt - 1);
+      w = APOD ? at[a] : DT(1);
+    } else {
+      ii = s / kGJ; jj = s % kGJ;
+      const int i = i0 + ii, j = j0 + jj;
+      ok = i < Lt && j < Lr;
+      const int ic = min(i, Lt - 1), jc = min(j, Lr - 1);
+      a = (TYPE == DAS_RCA_ROWTX) ? jc * Lt + ic : ic * Lr + jc;
+      w = APOD ? at[ic] * ar[jc] : DT(1);
+    }
+    if (!ok) continue;                           // uniform
+    const unsigned abase = (unsigned) a * (unsigned) K;
+    const int w0 = want_gy ? s_w0[s] : 0;
+    GT u[kRPU], tsum[kRPU];
+#pragma unroll
+    for (int q = 0; q < kRPU; q++) {
+      if (TYPE == DAS_SAFT) {
+        GT t = leg3<GT>(Gt[a] - xo[q], Gt[a + Lt] - yo[q], Gt[a + 2 * Lt] - zo[q]) * inv_cp;
         tsum[q] = t + t;
         t += dl;
         u[q] = clamp_leg<GT>(t + t, tmax) * fs;
       } else {
-        GT tt, tr;
-        if (TYPE == DAS_TFM) {
-          tt = leg3<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
-          tr = leg3<GT>(Gr[j] - xo, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
-        } else {
-          tt = leg3<GT>(Gt[i] - xo, stripe_gap<GT>(yo, y_lo, y_hi), Gt[i + 2 * Lt] - zo) * inv_cp;
-          tr = leg3<GT>(stripe_gap<GT>(xo, x_lo, x_hi), Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
-        }
-        tsum[q] = tt + tr;
+        const int i = min(i0 + ii, Lt - 1);
+        GT tt;
+        if (TYPE == DAS_TFM)
+          tt = leg3<GT>(Gt[i] - xo[q], Gt[i + Lt] - yo[q], Gt[i + 2 * Lt] - zo[q]) * inv_cp;
+        else
+          tt = leg3<GT>(Gt[i] - xo[q], gy_c[q], Gt[i + 2 * Lt] - zo[q]) * inv_cp;
+        tsum[q] = tt + tr[jj][q];
         tt += dl;
-        u[q] = (clamp_leg<GT>(tt, tmax) + clamp_leg<GT>(tr, tmax)) * fs;
+        u[q] = (clamp_leg<GT>(tt, tmax) + clamp_leg<GT>(tr[jj][q], tmax)) * fs;
       }
     }
-    DT y0[kPU], y1[kPU], f[kPU];
-    int k[kPU];
+    DT y0[kRPU], y1[kRPU], f[kRPU];
+    int k[kRPU];
 #pragma unroll
-    for (int q = 0; q < kPU; q++)
+    for (int q = 0; q < kRPU; q++)
       fetch_taps<DT, GT, INTERP>(ry, abase, K, u[q], live[q], y0[q], y1[q], f[q], k[q]);
 #pragma unroll
-    for (int q = 0; q < kPU; q++) {
+    for (int q = 0; q < kRPU; q++) {
       const DT slope = INTERP == DREAM_HIP_LINEAR ? y1[q] - y0[q] : DT(0);
-      const DT val = y0[q] + f[q] * slope;
-      z += (double) (gp[q] * val);
-      if (INTERP == DREAM_HIP_LINEAR)
-        zc += (double) (gp[q] * w * slope) * ((double) tsum[q] * dudc_scale);
+      zsum[s] += gp[q] * (y0[q] + f[q] * slope);
+      if (INTERP == DREAM_HIP_LINEAR) zc += gp[q] * w * slope * (DT) tsum[q];
       if (want_gy && k[q] > -2) {
         const DT gw = gp[q] * w;
         if (INTERP == DREAM_HIP_NEAREST) {
-          atomicAdd(&gacc[k[q]], gw);
-          kmin = min(kmin, k[q]);
-          kmax = max(kmax, k[q]);
+          const unsigned l = (unsigned) (k[q] - w0);
+          if (l < (unsigned) kWin) lds_atomic_add(&win[s * kWin + l], (double) gw);
+          else unsafeAtomicAdd(&gy_frame[abase + k[q]], gw);
         } else {
-          if (k[q] >= 0) atomicAdd(&gacc[k[q]], gw * (DT(1) - f[q]));
-          if (k[q] + 1 < K) atomicAdd(&gacc[k[q] + 1], gw * f[q]);
-          kmin = min(kmin, max(k[q], 0));
-          kmax = max(kmax, min(k[q] + 1, K - 1));
+          const unsigned l = (unsigned) (k[q] - w0);
+          const DT v0 = gw * (DT(1) - f[q]), v1 = gw * f[q];
+          if (l + 1u < (unsigned) kWin) {         // both taps in the window
+            lds_atomic_add(&win[s * kWin + l], (double) v0);
+            lds_atomic_add(&win[s * kWin + l + 1], (double) v1);
+          } else {
+            if (k[q] >= 0) unsafeAtomicAdd(&gy_frame[abase + k[q]], v0);
+            if (k[q] + 1 < K) unsafeAtomicAdd(&gy_frame[abase + k[q] + 1], v1);
+          }
         }
       }
     }
   }
-  // Block reductions.
-  z = wave_sum_d(z);
-  zc = wave_sum_d(zc);
-  if ((threadIdx.x & 63) == 0) {
-    s_red[0][threadIdx.x >> 6] = z;
-    s_red[1][threadIdx.x >> 6] = zc;
+
+  // Per-A-scan sums and the cp sum: wave, then work-group, then one atomic.
+#pragma unroll
+  for (int s = 0; s < NA; s++) {
+    const double v = wave_sum_d((double) zsum[s]);
+    if (lane == 0) s_z[s][wave] = v;
   }
-  if (want_gy) {
-    atomicMin(&s_kmin, kmin);
-    atomicMax(&s_kmax, kmax);
+  {
+    const double v = wave_sum_d((double) zc);
+    if (lane == 0) s_zc[wave] = v;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double zs = 0.0, cs = 0.0;
-    for (int q = 0; q < kThreads / 64; q++) { zs += s_red[0][q]; cs += s_red[1][q]; }
-    atomicAdd(&((double *) A.z_t)[a], zs);
-    atomicAdd((double *) A.z_cp, cs);
+  if (threadIdx.x < NA) {
+    const int s = threadIdx.x;
+    int a = -1;
+    if (TYPE == DAS_SAFT) { if (n0 + s < Lt) a = n0 + s; }
+    else {
+      const int i = i0 + s / kGJ, j = j0 + s % kGJ;
+      if (i < Lt && j < Lr) a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
+    }
+    double v = 0.0;
+    for (int w = 0; w < kThreads / 64; w++) v += s_z[s][w];
+    if (a >= 0 && v != 0.0) atomicAdd(&((double *) A.z_t)[a], v);
   }
+  if (threadIdx.x == 0 && INTERP == DREAM_HIP_LINEAR) {
+    double v = 0.0;
+    for (int w = 0; w < kThreads / 64; w++) v += s_zc[w];
+    v *= -(double) A.fs_khz / A.cp;             // du/dcp = -(travel time) fs / cp
+    if (v != 0.0) atomicAdd((double *) A.z_cp, v);
+  }
+  // Windows into gY.
   if (want_gy) {
-    DT *gy = (DT *) A.gY + frame * A.y_stride + a * K;
-    if (blocks_per_channel == 1) {
-      for (int kk = threadIdx.x; kk < K; kk += kThreads) gy[kk] = gacc[kk];
-    } else {
-      for (int kk = s_kmin + threadIdx.x; kk <= s_kmax; kk += kThreads) {
-        const DT v = gacc[kk];
-        if (v != DT(0)) unsafeAtomicAdd(&gy[kk], v);
+    for (int q = threadIdx.x; q < NA * kWin; q += kThreads) {
+      const int s = q / kWin;
+      const DT v = (DT) win[q];
+      if (v == DT(0)) continue;
+      int a;
+      if (TYPE == DAS_SAFT) a = n0 + s;
+      else {
+        const int i = i0 + s / kGJ, j = j0 + s % kGJ;
+        a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
       }
+      const int kk = s_w0[s] + (q - s * kWin);
+      if (kk < 0 || kk >= K) continue;           // a linear tap just outside the A-scan
+      unsafeAtomicAdd(&gy_frame[(int64_t) a * K + kk], v);
     }
   }
 }
@@ -638,7 +817,7 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
   }
 }
 
-// From per-A-scan sums Z[a] (channel kernel) to the apodization gradients:
+// From per-A-scan sums Z[a] (region kernel) to the apodization gradients:
 // SAFT g_t[n] = Z[n]; two-leg g_t[i] = sum_j w_j Z[i,j], g_r[j] = sum_i w_i Z[i,j].
 template <typename DT>
 __global__ void adj_finalize_channel(int type, int Lt, int Lr, const double *Z, const DT *at,
@@ -780,18 +959,22 @@ void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t
 {
   const int64_t n_ascans = (TYPE == DAS_SAFT) ? a.Lt : (int64_t) a.Lt * a.Lr;
   if (TYPE == DAS_SAFT || want_gy) {
-    // Channel-major: enough blocks per A-scan to fill the device.
-    const int64_t target = (int64_t) cu_count() * 8;
-    int64_t bpc = std::max<int64_t>(1, target / std::max<int64_t>(n_ascans * a.n_frames, 1));
-    bpc = std::min<int64_t>(bpc, std::max<int64_t>(1, (a.No + kThreads - 1) / kThreads));
-    const int64_t ppb = (a.No + bpc - 1) / bpc;
-    bpc = (a.No + ppb - 1) / std::max<int64_t>(ppb, 1);
-    if (bpc < 1) bpc = 1;
-    dim3 grid((unsigned) bpc, (unsigned) n_ascans, (unsigned) a.n_frames);
-    const size_t lds = want_gy ? (size_t) a.K * sizeof(DT) : 0;
-    if (want_gy && bpc > 1) zero_async(a.gY, (size_t) (a.n_frames * a.y_stride) * sizeof(DT), s);
-    hipLaunchKernelGGL((das_adj_channel_kernel<DT, GT, TYPE, INTERP, APOD>), grid, dim3(kThreads),
-                       lds, s, a, perm, ppb, (int) bpc);
+    // Region-major: (pixel region, A-scan group, frame) work-groups.
+    const int64_t regions = (a.No + kRegion - 1) / kRegion;
+    int n_gj = 1;
+    int64_t groups;
+    if (TYPE == DAS_SAFT) {
+      groups = (a.Lt + kSaftGroup - 1) / kSaftGroup;
+    } else {
+      n_gj = (a.Lr + kGJ - 1) / kGJ;
+      groups = (int64_t) ((a.Lt + kGI - 1) / kGI) * n_gj;
+    }
+    const int na = TYPE == DAS_SAFT ? kSaftGroup : kGI * kGJ;
+    const size_t lds = want_gy ? (size_t) na * kWin * sizeof(double) : 0;
+    if (want_gy) zero_async(a.gY, (size_t) (a.n_frames * a.y_stride) * sizeof(DT), s);
+    dim3 grid((unsigned) regions, (unsigned) groups, (unsigned) a.n_frames);
+    hipLaunchKernelGGL((das_adj_region_kernel<DT, GT, TYPE, INTERP, APOD>), grid,
+                       dim3(kThreads), lds, s, a, perm, n_gj);
   } else {
     const int64_t tiles = ((a.No + kThreads - 1) / kThreads) * a.n_frames;
     const int grid = (int) std::min<int64_t>(tiles, (int64_t) cu_count() * 4);
@@ -1114,11 +1297,9 @@ void DasHandle::adjoint_dev(const void *Y, const void *Ro, const void *Gt, const
   const int64_t n_ascans = type_ == DAS_SAFT ? Lt_ : Lt_ * Lr_;
   if (y_stride <= 0) y_stride = n_ascans * K_;
   if (im_stride <= 0) im_stride = No_;
-  require(!gY || (size_t) K_ * elem_size(prec_) <= 65536,
-          "das adjoint: the Y-gradient needs K * sizeof(T) <= 64 KiB (one A-scan in LDS)");
   require(type_ == DAS_SAFT || gY || (size_t) (Lt_ + Lr_) * 8 <= 65536,
           "das adjoint: too many elements for the apodization accumulators");
-  // Accumulators: Z per A-scan (channel kernel) or z_t/z_r (pixel kernel), z_cp.
+  // Accumulators: Z per A-scan (region kernel) or z_t/z_r (pixel kernel), z_cp.
   const size_t zbytes = (size_t) (n_ascans + Lt_ + Lr_ + 1) * sizeof(double);
   if (zbytes > d_z_bytes_) {
     (void) hipFree(d_z_);
