@@ -65,17 +65,20 @@ hipStream_t lib_stream();
 // root (relative error about 2^-23) and two Goldschmidt steps.  Not
 // correctly rounded: callers use it only where a certified error bound
 // decides the result, and fall back to the IEEE sqrt otherwise.
+// STEPS = 2 (the default) leaves a relative error of a few ulp; STEPS = 1
+// leaves about 1.5 e0^2, e0 being the seed's relative error.
+template <int STEPS = 2>
 __device__ __forceinline__ void sqrt_rsqrt(double s, double &r, double &rinv)
 {
   double y = __builtin_amdgcn_rsq(s);
   double g = s * y;
   double h = 0.5 * y;
-  double e = __builtin_fma(-g, h, 0.5);
-  g = __builtin_fma(g, e, g);
-  h = __builtin_fma(h, e, h);
-  e = __builtin_fma(-g, h, 0.5);
-  g = __builtin_fma(g, e, g);
-  h = __builtin_fma(h, e, h);
+#pragma unroll
+  for (int k = 0; k < STEPS; k++) {
+    const double e = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+  }
   r = g;
   rinv = h + h;
 }

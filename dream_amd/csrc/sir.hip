@@ -21,12 +21,16 @@
 // a wave see rows spread over the whole element and hit different bins.
 //
 // Sample index.  The index must equal the reference's FP64 sequence bit for
-// bit.  A fast estimate u = r*c1 - d/dt (r from the hardware rsqrt and two
-// Goldschmidt steps, c1 = 1000/(cp dt)) differs from the reference's
-// pre-rounding value by far less than 2^-40 (|u| + |d/dt| + 1); when u's
-// fractional part is farther than that from 1/2, rint of both agree.  Only
-// the other contributions (about one in 10^6) take the exact IEEE path:
-// correctly rounded sqrt, two correctly rounded divisions and rint.
+// bit.  A fast estimate u = r*c1 - d/dt (r from the hardware rsqrt, about
+// 2^-24, and one Goldschmidt step, about 2^-47; c1 = 1000/(cp dt)) differs
+// from the reference's pre-rounding value by less than 2^-44 (|u| + |d/dt|
+// + 1), against a certification margin of 2^-36 (same terms); rint(u) is
+// taken when u lies farther than B = 2^-36 (nt + 2 + |d/dt|) from a
+// rounding boundary (half-integer), which certifies it for every |u| <=
+// nt + 2, and for larger |u| both values are out of range anyway.  The
+// other contributions (about one in 10^7), and s = 0 or non-finite values,
+// take the exact IEEE path: correctly rounded sqrt, two correctly rounded
+// divisions and rint.
 
 #include <algorithm>
 #include <cmath>
@@ -40,21 +44,19 @@ namespace dream {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr double kBoundScale = 0x1p-40;
+constexpr double kBoundScale = 0x1p-36;
 
-// Sample index (as a double) and 1/r for squared distance s; see the file
-// comment for the error argument.
-__device__ __forceinline__ double sir_index(double s, double c1, double c2, double cbound,
+// Sample index (as a double) and 1/r for squared distance s.  `lim` is
+// 0.5 - B (see the file comment): |u - rint(u)| <= lim certifies rint(u).
+template <int STEPS>
+__device__ __forceinline__ double sir_index(double s, double c1, double c2, double lim,
                                             double cp, double d, double dt, double &rinv)
 {
   double r;
-  sqrt_rsqrt(s, r, rinv);
+  sqrt_rsqrt<STEPS>(s, r, rinv);
   const double u = __builtin_fma(r, c1, -c2);
-  const double fl = __builtin_floor(u);
-  const double fr = u - fl;
-  const double bound = __builtin_fma(__builtin_fabs(u), kBoundScale, cbound);
-  double ru = fl + (fr > 0.5 ? 1.0 : 0.0);
-  if (!(s > 0x1p-900 && __builtin_fabs(fr - 0.5) > bound)) {
+  double ru = __builtin_rint(u);
+  if (!(__builtin_fabs(u - ru) <= lim)) {        // also s = 0, NaN, inf
     const double re = __builtin_sqrt(s);
     const double t = (re * 1000.0) / cp;
     ru = __builtin_rint((t - d) / dt);
@@ -121,11 +123,18 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
   int cur = -1;
   double acc = 0.0;
   const double *el = P.el;
+  // Rows q = e*ny + j are dealt to lanes with stride kThreads; (e, j, row)
+  // advance incrementally (row = j*row_stride mod ny, unchanged by j -= ny).
+  int q = q0 + (int) threadIdx.x;
+  int e = q / ny;
+  int j = q - e * ny;
+  int row = (int) (((int64_t) j * P.row_stride) % ny);
+  // One Goldschmidt step: gfx950's FP64 rsqrt seed is accurate to about
+  // 2^-24, which one step takes to about 2^-47 -- enough for the index
+  // certification and for FP64 amplitudes.
+  constexpr int kSteps = 1;
 
-  for (int q = q0 + (int) threadIdx.x; q < q1; q += kThreads) {
-    const int e = q / ny;
-    const int j = q - e * ny;
-    const int row = (int) (((int64_t) j * P.row_stride) % ny);
+  for (; q < q1; q += kThreads) {
     const double xe = xo - el[e];
     const double ye = yo - el[L + e];
     const double ze = zo - el[2 * L + e];
@@ -134,14 +143,14 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
     const double ryy = ry * ry;
     const double rzz = ze * ze;
     const double c2 = d * P.inv_dt;
-    const double cbound = kBoundScale * (__builtin_fabs(c2) + 1.0);
+    const double lim = 0.5 - kBoundScale * (ntd + 2.0 + __builtin_fabs(c2));
     const double amp = el[5 * L + e] * P.amp;
     const bool last = (e == L - 1);
     for (int i = 0; i < nx; i++) {
       const double rx = xe - P.xs[i];
       const double s = (rx * rx + ryy) + rzz;
       double rinv;
-      const double ru = sir_index(s, P.c1, c2, cbound, P.cp, d, P.dt, rinv);
+      const double ru = sir_index<kSteps>(s, P.c1, c2, lim, P.cp, d, P.dt, rinv);
       const double ai = amp * rinv;
       if (ru >= 0.0 && ru < ntd) {
         const int bin = (int) ru;
@@ -159,6 +168,11 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
         oob_last |= last;
       }
     }
+    e += P.step_e;
+    j += P.step_j;
+    if (j >= ny) { j -= ny; e++; }
+    row += P.step_row;
+    if (row >= ny) row -= ny;
   }
   if (cur >= 0) {
     if (use_lds) lds_add(hist + cur, acc); else global_add(out + cur, (T) acc);
@@ -244,9 +258,12 @@ __global__ void rect_indices_kernel(double xo, double yo, double zo, const doubl
   const double s = (rx * rx + ry * ry) + zo * zo;
   const double c1 = 1000.0 / (cp * dt);
   const double c2 = delay * (1.0 / dt);
-  const double cbound = kBoundScale * (__builtin_fabs(c2) + 1.0);
-  double rinv;
-  it_out[q] = (int64_t) sir_index(s, c1, c2, cbound, cp, delay, dt, rinv);
+  // Every index is wanted here, so the bound covers this contribution's |u|.
+  double r, rinv;
+  sqrt_rsqrt<1>(s, r, rinv);
+  const double umax = __builtin_fabs(__builtin_fma(r, c1, -c2));
+  const double lim = 0.5 - kBoundScale * (umax + 2.0 + __builtin_fabs(c2));
+  it_out[q] = (int64_t) sir_index<1>(s, c1, c2, lim, cp, delay, dt, rinv);
 }
 
 int gcd_int(int a, int b)
@@ -454,6 +471,9 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
       while (gcd_int(stride, (int) ny) != 1) stride++;
     }
     d.row_stride = stride;
+    d.step_e = (int32_t) (kThreads / ny);
+    d.step_j = (int32_t) (kThreads % ny);
+    d.step_row = (int32_t) (((int64_t) d.step_j * stride) % ny);
     d.nt = p.nt;
     d.cp = p.cp;
     d.dt = p.dt;

@@ -30,6 +30,7 @@ struct SirDev {
   const double *delay;  int32_t delay_multi;
   int32_t L;
   const double *xs;  const double *ys;  int32_t nx, ny, row_stride, chunks;
+  int32_t step_e, step_j, step_row, pad0;   // row walk: q += 256 as (e, j, row) steps
   const double *el;                 // 6 x L: gx gy gz foc steer num
   int64_t nt;
   double cp, dt, inv_dt, c1, amp;   // c1 = 1000/(cp dt), amp = 1000/(2 pi dt)
