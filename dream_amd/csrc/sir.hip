@@ -65,6 +65,13 @@ __device__ __forceinline__ double sir_index(double s, double c1, double c2, doub
   return ru;
 }
 
+// Address-space views of the plan's tables: uniform reads of the
+// sub-element x table go through the scalar cache (constant space), and
+// per-lane table reads are global loads, which -- unlike generic (flat)
+// loads -- do not also hold up waits on LDS operations.
+typedef const __attribute__((address_space(4))) double cdouble;
+typedef const __attribute__((address_space(1))) double gdouble;
+
 template <typename T>
 __device__ __forceinline__ void lds_add(T *p, T v)
 {
@@ -102,8 +109,9 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
 {
   const int64_t n = local / P.chunks;
   const int c = (int) (local - n * P.chunks);
-  const double xo = P.Ro[n], yo = P.Ro[n + P.No], zo = P.Ro[n + 2 * P.No];
-  const double dlay = P.delay_multi ? P.delay[n] : P.delay[0];
+  gdouble *Ro = (gdouble *) P.Ro;
+  const double xo = Ro[n], yo = Ro[n + P.No], zo = Ro[n + 2 * P.No];
+  const double dlay = P.delay_multi ? ((gdouble *) P.delay)[n] : ((gdouble *) P.delay)[0];
   const int ny = P.ny, nx = P.nx, L = P.L;
   const int Q = L * ny;
   const int q0 = (int) ((int64_t) c * Q / P.chunks);
@@ -122,7 +130,9 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
   bool oob_last = false;
   int cur = -1;
   double acc = 0.0;
-  const double *el = P.el;
+  gdouble *el = (gdouble *) P.el;
+  gdouble *ys = (gdouble *) P.ys;
+  cdouble *xs = (cdouble *) P.xs;
   // Rows q = e*ny + j are dealt to lanes with stride kThreads; (e, j, row)
   // advance incrementally (row = j*row_stride mod ny, unchanged by j -= ny).
   int q = q0 + (int) threadIdx.x;
@@ -139,15 +149,15 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
     const double ye = yo - el[L + e];
     const double ze = zo - el[2 * L + e];
     const double d = (dlay - el[3 * L + e]) - el[4 * L + e];
-    const double ry = ye - P.ys[row];
+    const double ry = ye - ys[row];
     const double ryy = ry * ry;
     const double rzz = ze * ze;
     const double c2 = d * P.inv_dt;
     const double lim = 0.5 - kBoundScale * (ntd + 2.0 + __builtin_fabs(c2));
     const double amp = el[5 * L + e] * P.amp;
     const bool last = (e == L - 1);
-    for (int i = 0; i < nx; i++) {
-      const double rx = xe - P.xs[i];
+    auto contribution = [&](double x) {
+      const double rx = xe - x;
       const double s = (rx * rx + ryy) + rzz;
       double rinv;
       const double ru = sir_index<kSteps>(s, P.c1, c2, lim, P.cp, d, P.dt, rinv);
@@ -167,7 +177,18 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
         oob++;
         oob_last |= last;
       }
+    };
+    // Sub-element x positions four at a time, so their scalar loads share
+    // one wait.
+    int i = 0;
+    for (; i + 4 <= nx; i += 4) {
+      const double x0 = xs[i], x1 = xs[i + 1], x2 = xs[i + 2], x3 = xs[i + 3];
+      contribution(x0);
+      contribution(x1);
+      contribution(x2);
+      contribution(x3);
     }
+    for (; i < nx; i++) contribution(xs[i]);
     e += P.step_e;
     j += P.step_j;
     if (j >= ny) { j -= ny; e++; }
