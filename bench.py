@@ -26,6 +26,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+# HBM bytes per launch of the headline kernel, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE and scaled by a device copy of known size.
+COUNTERS = os.path.join(ROOT, "profiles", "das_tfm_c2_counters.md")
+
+
+def measured_traffic():
+    """traffic_bytes_per_launch from the committed counter summary, or None."""
+    try:
+        with open(COUNTERS) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 METRIC = "DAS pixel·channel sums/s + SIR obs·elem/s vs FP64 CPU; %HBM roofline; bwd/fwd"
 
 
@@ -302,7 +317,8 @@ def main():
                                "sample (reference rule)",
                    "n_pixels": 65536, "n_channels": 4096, "in_bounds_pairs": res["n_inb"]},
         "roofline": {"bound": "hbm", "achieved": res["achieved"] / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": res["achieved"] / HBM_PEAK, "traffic": None},
+                     "unit": "GB/s", "frac": res["achieved"] / HBM_PEAK,
+                     "traffic": measured_traffic()},
         "cpu_baseline": cpu,
     }
     if not args.no_extra:
