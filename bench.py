@@ -180,6 +180,52 @@ def bench_sir(steps, warmup):
     return out
 
 
+def bench_batches(steps, warmup):
+    """Uneven batches of small problems, where one persistent launch matters:
+    96 TFM images (8-48 elements, A-scans of 512-2048 samples, 16^2-80^2 pixels)
+    and 128 SIR problems (16-256 points, 4-16 elements, nt 200-800), each in one
+    launch (captured graph) against one plain call per problem."""
+    import numpy as np
+    import torch
+
+    import workloads
+    from dream_amd.torch_ops import DasBatch, DasPlan, SirBatch, arr_rect
+    out = {}
+    probs = workloads.das_uneven_batch(device="cuda")
+    batch = DasBatch("tfm", probs, precision="f32")
+    plans = [DasPlan("tfm", p["K"], p["Ro"], p["Gt"], p["Gr"], precision="f32") for p in probs]
+    outs = [torch.empty(pl.No, dtype=torch.float32, device="cuda") for pl in plans]
+
+    def per_call():
+        for p, pl, o in zip(probs, plans, outs):
+            pl.forward(p["Y"], p["dt"], 0.0, p["cp"], out=o)
+    ms_b = time_launches(lambda: batch.run(use_graph=True), steps, warmup)
+    ms_p = time_launches(per_call, max(5, steps // 5), 2)
+    pairs = sum(pl.No * pl.n_ascans for pl in plans)
+    out["das_uneven_batch_f32"] = dict(problems=len(probs), ms=round(ms_b, 4),
+                                       per_call_ms=round(ms_p, 4),
+                                       speedup_vs_per_call=ms_p / ms_b,
+                                       sums_per_s=pairs / (ms_b * 1e-3))
+    sprobs = workloads.sir_small_uneven_batch()
+    dev = [_sir_device_problem(p, torch) for p in sprobs]
+    sb = SirBatch(dev, out_dtype=torch.float32)
+    Hs = [torch.empty_like(H) for H in sb.H]
+
+    def sir_per_call():
+        for p, q, H in zip(sprobs, dev, Hs):
+            arr_rect(q["Ro"], p["a"], p["b"], p["dx"], p["dy"], p["dt"], p["nt"], q["delay"],
+                     p["v"], p["cp"], p["G"], foc_met="ud", focal=p["focal"], apod_met="hann",
+                     err_level="ignore", out_dtype=torch.float32, out=H)
+    ms_b = time_launches(lambda: sb.run(use_graph=True), steps, warmup)
+    ms_p = time_launches(sir_per_call, max(2, steps // 25), 1)
+    n_oe = sum(p["Ro"].shape[0] * p["G"].shape[0] for p in sprobs)
+    out["sir_small_uneven_batch_f32"] = dict(problems=len(sprobs), ms=round(ms_b, 4),
+                                             per_call_ms=round(ms_p, 4),
+                                             speedup_vs_per_call=ms_p / ms_b,
+                                             obs_elem_per_s=n_oe / (ms_b * 1e-3))
+    return out
+
+
 def bench_das_diff(steps, warmup):
     """C4 SAFT (linear, apodized) forward and adjoint; C5 RCA 8 frames forward and
     apodization adjoint."""
@@ -267,9 +313,10 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: headline kernel only, plus a 256 MiB "
                          "device-to-device copy used to calibrate the HBM byte counters")
-    ap.add_argument("--part", choices=("sir", "c4", "c5"),
+    ap.add_argument("--part", choices=("sir", "c4", "c5", "batch"),
                     help="profile one of the other paths alone (short run, no JSON line): "
-                         "sir = C1 + C3 SIR, c4 = SAFT forward/adjoint, c5 = RCA frames")
+                         "sir = C1 + C3 SIR, c4 = SAFT forward/adjoint, c5 = RCA frames, "
+                         "batch = uneven batches of small DAS and SIR problems")
     args = ap.parse_args()
     if args.profile:
         args.no_cpu = True
@@ -283,7 +330,8 @@ def main():
     from dream_amd import _lib
     _lib.require_device()
     if args.part:
-        part = {"sir": bench_sir, "c4": bench_das_diff_c4, "c5": bench_das_diff_c5}[args.part]
+        part = {"sir": bench_sir, "c4": bench_das_diff_c4, "c5": bench_das_diff_c5,
+                "batch": bench_batches}[args.part]
         log(json.dumps(part(args.steps, args.warmup)))
         return
 
@@ -298,6 +346,7 @@ def main():
         log("calibration copy: 3 x 268435456 bytes")
     if not args.no_extra:
         extra.update(bench_sir(args.steps, args.warmup))
+        extra.update(bench_batches(args.steps, args.warmup))
         extra.update(bench_das_diff(args.steps, args.warmup))
     line = {
         "metric": METRIC,

@@ -33,6 +33,7 @@ EXPORTS = (
     "dream_hip_das_create", "dream_hip_das_destroy", "dream_hip_das_setup_has_changed",
     "dream_hip_das_set_pixel_order", "dream_hip_pixel_order",
     "dream_hip_das", "dream_hip_das_dev", "dream_hip_das_adjoint_dev",
+    "dream_hip_das_batch_create", "dream_hip_das_batch_run", "dream_hip_das_batch_destroy",
 )
 
 _i32, _i64, _dbl = ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -57,6 +58,18 @@ class DreamSirProblem(ctypes.Structure):
         ("apod_par", _dbl),
         ("err_level", ctypes.c_int32),
         ("h", _vp),
+    ]
+
+
+class DreamDasProblem(ctypes.Structure):
+    """Mirror of the DreamDasProblem struct of include/dream_hip.h."""
+    _fields_ = [
+        ("K", _i64), ("No", _i64), ("Lt", _i64), ("Lr", _i64),
+        ("Y", _vp), ("Ro", _vp), ("Gt", _vp), ("Gr", _vp),
+        ("dt", _dbl), ("delay", _dbl), ("cp", _dbl),
+        ("apod_t", _vp), ("apod_r", _vp),
+        ("perm", _vp),
+        ("Im", _vp),
     ]
 
 
@@ -112,6 +125,13 @@ def _declare(L):
     L.dream_hip_das_dev.restype = _i32
     L.dream_hip_das_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _dbl, _i32, _vp, _dbl, _vp, _vp,
                                     _i32, _i64, _i64, _i64, _vp, _i32, _ip, _vp]
+    L.dream_hip_das_batch_create.restype = _i32
+    L.dream_hip_das_batch_create.argtypes = [ctypes.POINTER(DreamDasProblem), ctypes.c_int32,
+                                             _i32, _i32, _i32, ctypes.POINTER(_vp)]
+    L.dream_hip_das_batch_run.restype = _i32
+    L.dream_hip_das_batch_run.argtypes = [_vp, _vp, _i32]
+    L.dream_hip_das_batch_destroy.restype = _i32
+    L.dream_hip_das_batch_destroy.argtypes = [_vp]
     L.dream_hip_das_adjoint_dev.restype = _i32
     L.dream_hip_das_adjoint_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _vp, _vp,
                                             _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]

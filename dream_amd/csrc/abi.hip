@@ -358,4 +358,31 @@ int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const
   });
 }
 
+int dream_hip_das_batch_create(const DreamDasProblem *problems, int32_t n, int das_type,
+                               int precision, int interp, void **batch)
+{
+  return guarded([&]() {
+    require(batch != nullptr && (problems || n == 0) && n >= 0, "das_batch_create: bad arguments");
+    *batch = new DasBatch(problems, n, das_type, precision, interp);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_batch_run(void *batch, void *stream, int use_graph)
+{
+  return guarded([&]() {
+    require(batch != nullptr, "das_batch_run: null batch");
+    static_cast<DasBatch *>(batch)->run((hipStream_t) stream, use_graph != 0);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_batch_destroy(void *batch)
+{
+  return guarded([&]() {
+    delete static_cast<DasBatch *>(batch);
+    return DREAM_HIP_OK;
+  });
+}
+
 } // extern "C"

@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <utility>
 #include <vector>
 
@@ -191,19 +192,15 @@ constexpr int kSB = 8;     // SAFT elements per job (one unrolled block)
 
 constexpr int kMaxWaves = 8;     // waves per forward work-group (64 .. 512 threads)
 
+// The sum over the A-scans of jobs job0, job0 + jstride, ... of the
+// pixel at visiting position idx (one lane per pixel, 64 per wave; every
+// lane of the wave runs the same jobs).  `oob` is set when a pair of the
+// pixel fell outside [0, K) (only tracked when STATUS).
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD, bool STATUS>
-__global__ void __launch_bounds__(64 * kMaxWaves)
-das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__restrict__ partial)
+__device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__restrict__ perm,
+                                           int64_t idx, int64_t frame, int job0, int jstride,
+                                           bool &oob)
 {
-  __shared__ DT red[kMaxWaves][64];
-  const int nwaves = blockDim.x >> 6;
-  // The wave index is uniform across the wave; say so, so that job, element
-  // and A-scan indices (and the per-A-scan descriptors) live in SGPRs.
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t idx = (int64_t) blockIdx.x * 64 + lane;
-  const int sidx = blockIdx.y;
-  const int64_t frame = blockIdx.z;
   const bool valid = idx < A.No;
   const int64_t p = valid ? pixel_of(A, perm, idx) : 0;
   const GT *Ro = (const GT *) A.Ro;
@@ -221,13 +218,11 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
   const DT *at = (const DT *) A.apod_t;
   const DT *ar = (const DT *) A.apod_r;
   const int Lt = A.Lt, Lr = A.Lr;
-  const int stride = nwaves * split;
   DT acc = DT(0);
-  bool oob = false;
 
   if (TYPE == DAS_SAFT) {
     const int J = (Lt + kSB - 1) / kSB;
-    for (int job = sidx * nwaves + wave; job < J; job += stride) {
+    for (int job = job0; job < J; job += jstride) {
       const int n0 = job * kSB;
       GT u[kSB];
       bool any_in = false;
@@ -266,7 +261,7 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
     }
     const int nchunks = (Lr + kRC - 1) / kRC;
     const int J = nchunks * ((Lt + kTB - 1) / kTB);
-    for (int job = sidx * nwaves + wave; job < J; job += stride) {
+    for (int job = job0; job < J; job += jstride) {
       const int c = job % nchunks, b = job / nchunks;
       const int j0 = c * kRC, i0 = b * kTB, i1 = min(i0 + kTB, Lt);
       const bool full = j0 + kRC <= Lr;
@@ -334,6 +329,27 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
       }
     }
   }
+  return acc;
+}
+
+template <typename DT, typename GT, int TYPE, int INTERP, bool APOD, bool STATUS>
+__global__ void __launch_bounds__(64 * kMaxWaves)
+das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__restrict__ partial)
+{
+  __shared__ DT red[kMaxWaves][64];
+  const int nwaves = blockDim.x >> 6;
+  // The wave index is uniform across the wave; say so, so that job, element
+  // and A-scan indices live in SGPRs.
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t idx = (int64_t) blockIdx.x * 64 + lane;
+  const int sidx = blockIdx.y;
+  const int64_t frame = blockIdx.z;
+  const bool valid = idx < A.No;
+  const int64_t p = valid ? pixel_of(A, perm, idx) : 0;
+  bool oob = false;
+  const DT acc = fwd_tile_sum<DT, GT, TYPE, INTERP, APOD, STATUS>(
+      A, perm, idx, frame, sidx * nwaves + wave, nwaves * split, oob);
   // Sum the waves' partial sums in a fixed order.
   red[wave][lane] = acc;
   __syncthreads();
@@ -360,6 +376,54 @@ __global__ void das_reduce_kernel(const DT *__restrict__ partial, int split, int
   DT sum = partial[q];
   for (int s = 1; s < split; s++) sum += partial[(int64_t) s * n_frames * No + q];
   Im[f * im_stride + p] = sum;
+}
+
+// ---------------------------------------------------------------- uneven batch
+
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v)
+{
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned) v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned) (v >> 32));
+  return ((unsigned long long) hi << 32) | lo;
+}
+
+// Many problems of one type in one persistent launch.  A work item is
+// (problem, 64-pixel tile); items are numbered problem by problem, problems
+// in decreasing per-tile cost, and each WAVE pulls items from counter[0] and
+// runs every job of its tile, so uneven problems keep all waves busy until
+// the list drains.  The last wave to leave resets both counters, so the
+// launch can be captured into a hipGraph and replayed.
+template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
+__global__ void __launch_bounds__(kThreads)
+das_batch_kernel(const DasArgs *__restrict__ args, const int64_t *__restrict__ item_begin,
+                 int n_probs, int64_t n_items, unsigned long long *counter)
+{
+  const int lane = threadIdx.x & 63;
+  const unsigned long long total_waves = (unsigned long long) gridDim.x * (blockDim.x >> 6);
+  for (;;) {
+    unsigned long long got = 0;
+    if (lane == 0) got = atomicAdd(&counter[0], 1ull);
+    const int64_t item = (int64_t) uniform_u64(__shfl(got, 0, 64));
+    if (item >= n_items) break;
+    int lo = 0, hi = n_probs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (item_begin[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const DasArgs &A = args[lo];
+    const int64_t idx = (item - item_begin[lo]) * 64 + lane;
+    bool oob = false;
+    const DT acc = fwd_tile_sum<DT, GT, TYPE, INTERP, APOD, false>(A, A.perm, idx, 0, 0, 1, oob);
+    if (idx < A.No) ((DT *) A.Im)[pixel_of(A, A.perm, idx)] = acc;
+  }
+  if (lane == 0) {
+    __threadfence();
+    const unsigned long long done = atomicAdd(&counter[1], 1ull);
+    if (done == total_waves - 1) {
+      atomicExch(&counter[0], 0ull);
+      atomicExch(&counter[1], 0ull);
+    }
+  }
 }
 
 // Both taps of one pair for the adjoint: y0 = Y[k], y1 = Y[k+1] (0 outside
@@ -1011,6 +1075,48 @@ void adj_dispatch_type(const DasArgs &a, const int32_t *perm, int type, int inte
   }
 }
 
+template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
+void batch_launch(const DasArgs *args, const int64_t *begin, int n, int64_t items,
+                  unsigned long long *counter, int grid, hipStream_t s)
+{
+  hipLaunchKernelGGL((das_batch_kernel<DT, GT, TYPE, INTERP, APOD>), dim3(grid), dim3(kThreads), 0,
+                     s, args, begin, n, items, counter);
+}
+
+template <typename DT, typename GT, int TYPE>
+void batch_dispatch_interp(int interp, bool apod, const DasArgs *args, const int64_t *begin, int n,
+                           int64_t items, unsigned long long *counter, int grid, hipStream_t s)
+{
+  if (interp == DREAM_HIP_NEAREST) {
+    if (apod) batch_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, true>(args, begin, n, items, counter, grid, s);
+    else batch_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, false>(args, begin, n, items, counter, grid, s);
+  } else {
+    if (apod) batch_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, true>(args, begin, n, items, counter, grid, s);
+    else batch_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, false>(args, begin, n, items, counter, grid, s);
+  }
+}
+
+template <typename DT, typename GT>
+void batch_dispatch_type(int type, int interp, bool apod, const DasArgs *args,
+                         const int64_t *begin, int n, int64_t items,
+                         unsigned long long *counter, int grid, hipStream_t s)
+{
+  switch (type) {
+  case DAS_SAFT:
+    batch_dispatch_interp<DT, GT, DAS_SAFT>(interp, apod, args, begin, n, items, counter, grid, s);
+    break;
+  case DAS_TFM:
+    batch_dispatch_interp<DT, GT, DAS_TFM>(interp, apod, args, begin, n, items, counter, grid, s);
+    break;
+  case DAS_RCA_COLTX:
+    batch_dispatch_interp<DT, GT, DAS_RCA_COLTX>(interp, apod, args, begin, n, items, counter, grid, s);
+    break;
+  default:
+    batch_dispatch_interp<DT, GT, DAS_RCA_ROWTX>(interp, apod, args, begin, n, items, counter, grid, s);
+    break;
+  }
+}
+
 size_t elem_size(int prec) { return prec == DREAM_HIP_F64 ? 8 : 4; }
 size_t geom_size(int prec) { return prec == DREAM_HIP_F32_REF ? 4 : 8; }
 
@@ -1084,6 +1190,22 @@ void launch_das_adjoint(const DasArgs &a, int type, int prec, int interp, bool a
 }
 
 // ------------------------------------------------------------------ handle
+
+void das_constants(DasArgs &a, int prec, double dt, double cp)
+{
+  // The reference's constants: double expressions rounded to T
+  // (extra_src/das.cc:316-318).
+  if (prec == DREAM_HIP_F32_REF) {
+    a.fs_khz = (double) (float) ((1.0 / dt) * 1000.0);
+    a.inv_cp = (double) (float) (1.0 / cp);
+  } else {
+    a.fs_khz = (1.0 / dt) * 1000.0;
+    a.inv_cp = 1.0 / cp;
+  }
+  a.delay_ms = 0.0;
+  a.cp = cp;
+  a.leg_max = std::ldexp(1.0, 26) / a.fs_khz;
+}
 
 DasHandle::DasHandle(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, int precision)
   : type_(Lr == 0 ? DAS_SAFT : type), K_(K), No_(No), Lt_(Lt), Lr_(Lr), prec_(precision)
@@ -1163,19 +1285,7 @@ void DasHandle::fill_args(DasArgs &a, const void *Y, const void *Ro, const void 
   a.Lr = type_ == DAS_SAFT ? 0 : (int32_t) Lr_;
   a.delay = delay;
   a.delay_multi = delay_type != 0;
-  // The reference's constants: double expressions rounded to T
-  // (extra_src/das.cc:316-318).
-  if (prec_ == DREAM_HIP_F32_REF) {
-    a.fs_khz = (double) (float) ((1.0 / dt) * 1000.0);
-    a.inv_cp = (double) (float) (1.0 / cp);
-    a.delay_ms = 0.0;
-    a.cp = cp;
-  } else {
-    a.fs_khz = (1.0 / dt) * 1000.0;
-    a.inv_cp = 1.0 / cp;
-    a.cp = cp;
-  }
-  a.leg_max = std::ldexp(1.0, 26) / a.fs_khz;
+  das_constants(a, prec_, dt, cp);
   a.apod_t = apod_t;
   a.apod_r = apod_r;
   a.n_frames = n_frames;
@@ -1346,6 +1456,149 @@ void DasHandle::adjoint_dev(const void *Y, const void *Ro, const void *Gt, const
                          (float *) g_cp);
   }
   DREAM_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ batch
+
+DasBatch::DasBatch(const DreamDasProblem *probs, int32_t n, int type, int precision, int interp)
+  : type_(type), prec_(precision), interp_(interp)
+{
+  require(type >= DAS_SAFT && type <= DAS_RCA_ROWTX, "das batch: unknown DAS type");
+  require(precision == DREAM_HIP_F64 || precision == DREAM_HIP_F32_REF ||
+              precision == DREAM_HIP_F32,
+          "das batch: unknown precision");
+  require(interp == DREAM_HIP_NEAREST || interp == DREAM_HIP_LINEAR,
+          "das batch: unknown interpolation");
+  require(n >= 0 && (n == 0 || probs), "das batch: bad problem list");
+  const size_t es = elem_size(precision), gs = geom_size(precision);
+  std::vector<DasArgs> args((size_t) n);
+  std::vector<double> cost((size_t) n);
+  for (int32_t i = 0; i < n; i++) {
+    const DreamDasProblem &p = probs[i];
+    const bool saft = type == DAS_SAFT;
+    require(p.K >= 1 && p.K < (1ll << 26), "das batch: A-scan length out of range (1 .. 2^26)");
+    require(p.No >= 0 && p.Lt >= 1 && (saft || p.Lr >= 1), "das batch: bad array sizes");
+    const int64_t n_ascans = saft ? p.Lt : p.Lt * p.Lr;
+    require((uint64_t) (n_ascans * p.K) * es < 0xFFFFFFF0ull,
+            "das batch: Y of one problem must be smaller than 4 GiB");
+    require(p.No == 0 || (p.Y && p.Ro && p.Gt && p.Im && (saft || p.Gr)),
+            "das batch: null device pointer");
+    const bool apod = p.apod_t != nullptr;
+    require(!apod || saft || p.apod_r, "das batch: TFM/RCA apodization needs apod_r");
+    if (i == 0) apod_ = apod;
+    require(apod == apod_, "das batch: all problems or none must be apodized");
+    DasArgs &a = args[(size_t) i];
+    std::memset(&a, 0, sizeof a);
+    a.Y = p.Y;
+    a.K = p.K;
+    a.Ro = p.Ro;
+    a.No = p.No;
+    a.Gt = p.Gt;
+    a.Lt = (int32_t) p.Lt;
+    a.Gr = saft ? nullptr : p.Gr;
+    a.Lr = saft ? 0 : (int32_t) p.Lr;
+    das_constants(a, precision, p.dt, p.cp);
+    a.apod_t = p.apod_t;
+    a.apod_r = p.apod_r;
+    a.n_frames = 1;
+    a.y_stride = n_ascans * p.K;
+    a.im_stride = p.No;
+    a.Im = p.Im;
+    a.perm = p.perm;
+    cost[(size_t) i] = (double) n_ascans;
+  }
+  // Problems in decreasing per-tile cost; items numbered problem by problem.
+  std::vector<int32_t> order((size_t) n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t x, int32_t y) { return cost[(size_t) x] > cost[(size_t) y]; });
+  std::vector<DasArgs> sorted;
+  std::vector<int64_t> begin;
+  std::vector<double> delays;
+  int64_t item = 0;
+  for (int32_t k = 0; k < n; k++) {
+    const int32_t i = order[(size_t) k];
+    const int64_t tiles = (probs[i].No + 63) / 64;
+    if (tiles == 0) continue;
+    sorted.push_back(args[(size_t) i]);
+    begin.push_back(item);
+    delays.push_back(probs[i].delay);
+    item += tiles;
+  }
+  n_probs_ = (int32_t) sorted.size();
+  n_items_ = item;
+  const size_t nb = std::max<size_t>(sorted.size(), 1);
+  DREAM_HIP_CHECK(hipMalloc(&d_args_, nb * sizeof(DasArgs)));
+  DREAM_HIP_CHECK(hipMalloc(&d_begin_, nb * sizeof(int64_t)));
+  DREAM_HIP_CHECK(hipMalloc(&d_delay_, nb * sizeof(double)));
+  DREAM_HIP_CHECK(hipMalloc(&d_counter_, 2 * sizeof(unsigned long long)));
+  DREAM_HIP_CHECK(hipMemset(d_counter_, 0, 2 * sizeof(unsigned long long)));
+  // Scalar delays in the geometry type (float for the reference-float path).
+  if (gs == 8) {
+    if (!delays.empty())
+      DREAM_HIP_CHECK(hipMemcpy(d_delay_, delays.data(), delays.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(delays.begin(), delays.end());
+    if (!f.empty())
+      DREAM_HIP_CHECK(hipMemcpy(d_delay_, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  }
+  for (size_t k = 0; k < sorted.size(); k++) {
+    sorted[k].delay = (const char *) d_delay_ + k * gs;
+    sorted[k].delay_multi = 0;
+  }
+  if (!sorted.empty()) {
+    DREAM_HIP_CHECK(hipMemcpy(d_args_, sorted.data(), sorted.size() * sizeof(DasArgs),
+                              hipMemcpyHostToDevice));
+    DREAM_HIP_CHECK(hipMemcpy(d_begin_, begin.data(), begin.size() * sizeof(int64_t),
+                              hipMemcpyHostToDevice));
+  }
+  // Up to 8 waves per SIMD, never more waves than items.
+  const int64_t waves = std::min<int64_t>((int64_t) cu_count() * 32, n_items_);
+  grid_ = (int) std::max<int64_t>(1, (waves + 3) / 4);
+}
+
+DasBatch::~DasBatch()
+{
+  if (graph_) (void) hipGraphExecDestroy(graph_);
+  (void) hipFree(d_args_);
+  (void) hipFree(d_begin_);
+  (void) hipFree(d_delay_);
+  (void) hipFree(d_counter_);
+}
+
+void DasBatch::launch(hipStream_t s)
+{
+  if (n_items_ == 0) return;
+  if (prec_ == DREAM_HIP_F64)
+    batch_dispatch_type<double, double>(type_, interp_, apod_, d_args_, d_begin_, n_probs_,
+                                        n_items_, d_counter_, grid_, s);
+  else if (prec_ == DREAM_HIP_F32)
+    batch_dispatch_type<float, double>(type_, interp_, apod_, d_args_, d_begin_, n_probs_,
+                                       n_items_, d_counter_, grid_, s);
+  else
+    batch_dispatch_type<float, float>(type_, interp_, apod_, d_args_, d_begin_, n_probs_,
+                                      n_items_, d_counter_, grid_, s);
+  DREAM_HIP_CHECK(hipGetLastError());
+}
+
+void DasBatch::run(hipStream_t s, bool use_graph)
+{
+  if (cancelled()) throw Error(DREAM_HIP_ECANCELLED, "das batch cancelled");
+  if (!use_graph) {
+    launch(s);
+    return;
+  }
+  if (!graph_) {
+    // Capture on the library's own stream; the graph runs on the caller's.
+    hipStream_t cs = lib_stream();
+    hipGraph_t g = nullptr;
+    DREAM_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    launch(cs);
+    DREAM_HIP_CHECK(hipStreamEndCapture(cs, &g));
+    DREAM_HIP_CHECK(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+    DREAM_HIP_CHECK(hipGraphDestroy(g));
+  }
+  DREAM_HIP_CHECK(hipGraphLaunch(graph_, s));
 }
 
 } // namespace dream

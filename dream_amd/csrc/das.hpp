@@ -89,6 +89,31 @@ private:
   void ensure_scratch(size_t bytes);
 };
 
+// fs_khz, inv_cp, delay_ms (0), cp and leg_max of a launch, with the
+// reference's rounding of the constants to T (extra_src/das.cc:316-318).
+void das_constants(DasArgs &a, int prec, double dt, double cp);
+
+// Many problems of one DAS type, precision and sampling rule in one
+// persistent forward launch (device pointers; scalar delays).
+class DasBatch {
+public:
+  DasBatch(const DreamDasProblem *problems, int32_t n, int type, int precision, int interp);
+  ~DasBatch();
+  void run(hipStream_t s, bool use_graph);
+private:
+  void launch(hipStream_t s);
+  int type_, prec_, interp_;
+  bool apod_ = false;
+  int32_t n_probs_ = 0;
+  int64_t n_items_ = 0;
+  int grid_ = 1;
+  DasArgs *d_args_ = nullptr;
+  int64_t *d_begin_ = nullptr;
+  void *d_delay_ = nullptr;
+  unsigned long long *d_counter_ = nullptr;
+  hipGraphExec_t graph_ = nullptr;
+};
+
 void launch_das_forward(const DasArgs &a, int type, int prec, int interp, bool apod,
                         hipStream_t s);
 void launch_das_adjoint(const DasArgs &a, int type, int prec, int interp, bool apod,

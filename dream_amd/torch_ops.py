@@ -293,3 +293,69 @@ class SirBatch:
                 _lib._lib.dream_hip_sir_plan_destroy(self.plan)
         except Exception:
             pass
+
+
+class DasBatch:
+    """Many uneven DAS problems of one method, precision and sampling rule in
+    one persistent forward launch.
+
+    problems: list of dicts with keys K, Ro ((No, 3) array or tensor), Gt,
+    Gr (not for SAFT), Y ((n_ascans, K) device tensor of the batch's data
+    type), dt, cp, and optionally delay (scalar), apod_t and apod_r (device
+    tensors; all problems or none).  Outputs Im[i] ((No_i,) tensors) are
+    allocated by the batch; run() overwrites them.
+    """
+
+    def __init__(self, method, problems, precision="f32", interp="nearest", device="cuda",
+                 pixel_order=True):
+        _lib.require_device()
+        self.type = _lib.DAS_TYPE[method]
+        self.prec = PREC[precision]
+        self.data_dtype = torch.float64 if precision == "f64" else torch.float32
+        gd = torch.float32 if precision == "f32_ref" else torch.float64
+        dev = torch.device(device)
+        self.n = len(problems)
+        self._keep = []
+        self.Im = []
+        arr = (_lib.DreamDasProblem * max(self.n, 1))()
+        for i, p in enumerate(problems):
+            Ro = _dev_tensor(torch.as_tensor(p["Ro"], dtype=gd).reshape(-1, 3).t(), gd, dev)
+            Gt = _dev_tensor(torch.as_tensor(p["Gt"], dtype=gd).reshape(-1, 3).t(), gd, dev)
+            Gr = (None if self.type == 0 else
+                  _dev_tensor(torch.as_tensor(p["Gr"], dtype=gd).reshape(-1, 3).t(), gd, dev))
+            Y = p["Y"]
+            No, Lt = Ro.shape[1], Gt.shape[1]
+            Lr = 0 if Gr is None else Gr.shape[1]
+            n_ascans = Lt if self.type == 0 else Lt * Lr
+            assert Y.is_contiguous() and Y.dtype == self.data_dtype and \
+                tuple(Y.shape) == (n_ascans, int(p["K"])), "Y must be contiguous (n_ascans, K)"
+            perm = None
+            if pixel_order:
+                perm = torch.as_tensor(pixel_order_of(p["Ro"]), device=dev)
+            Im = torch.empty(No, dtype=self.data_dtype, device=dev)
+            self.Im.append(Im)
+            self._keep += [Ro, Gt, Gr, Y, perm, Im, p.get("apod_t"), p.get("apod_r")]
+            q = arr[i]
+            q.K, q.No, q.Lt, q.Lr = int(p["K"]), No, Lt, Lr
+            q.Y, q.Ro, q.Gt = Y.data_ptr(), Ro.data_ptr(), Gt.data_ptr()
+            q.Gr = None if Gr is None else Gr.data_ptr()
+            q.dt, q.delay, q.cp = float(p["dt"]), float(p.get("delay", 0.0)), float(p["cp"])
+            q.apod_t = None if p.get("apod_t") is None else p["apod_t"].data_ptr()
+            q.apod_r = None if p.get("apod_r") is None else p["apod_r"].data_ptr()
+            q.perm = None if perm is None else perm.data_ptr()
+            q.Im = Im.data_ptr()
+        self.batch = ctypes.c_void_p(None)
+        _lib.check(_lib.lib().dream_hip_das_batch_create(arr, self.n, self.type, self.prec,
+                                                         INTERP[interp], ctypes.byref(self.batch)))
+
+    def run(self, use_graph=True):
+        _lib.check(_lib.lib().dream_hip_das_batch_run(self.batch, _stream(),
+                                                      1 if use_graph else 0))
+        return self.Im
+
+    def __del__(self):
+        try:
+            if self.batch.value is not None and _lib._lib is not None:
+                _lib._lib.dream_hip_das_batch_destroy(self.batch)
+        except Exception:
+            pass

@@ -236,6 +236,34 @@ int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const
                               int64_t im_frame_stride, const void *g, void *gY,
                               void *g_apod_t, void *g_apod_r, void *g_cp, void *stream);
 
+/* Batched uneven DAS: many beamforming problems of one DAS type, precision
+ * and sampling rule -- different arrays, pixel grids and A-scan lengths --
+ * in one persistent forward launch.  No counterpart in the reference, whose
+ * gateways beamform one problem per DAS<T>::das call (extra_src/das.cc:
+ * 199-298); each problem has the meaning of one dream_hip_das_dev call with
+ * one frame and a scalar delay.  All pointers are DEVICE pointers of the
+ * batch's element type (geometry in double for DREAM_HIP_F32); apod_t /
+ * apod_r must be given for all problems or for none; perm (No entries) is
+ * the pixel visiting order or NULL. */
+typedef struct DreamDasProblem {
+  int64_t K, No, Lt, Lr;                     /* Lr is ignored for SAFT */
+  const void *Y, *Ro, *Gt, *Gr;              /* K x n_ascans, No x 3, Lt x 3, Lr x 3 */
+  double dt, delay, cp;                      /* us, us, m/s */
+  const void *apod_t, *apod_r;
+  const int32_t *perm;
+  void *Im;                                  /* No, overwritten */
+} DreamDasProblem;
+
+/* Build a batch (device copies of the descriptors, work list of 64-pixel
+ * tiles in decreasing per-tile cost).  The device arrays must stay valid
+ * while the batch is used. */
+int dream_hip_das_batch_create(const DreamDasProblem *problems, int32_t n, int das_type,
+                               int precision, int interp, void **batch);
+/* Launch on `stream` (captured into a hipGraph on first use when use_graph
+ * != 0, replayed afterwards).  Asynchronous; no out-of-bounds status. */
+int dream_hip_das_batch_run(void *batch, void *stream, int use_graph);
+int dream_hip_das_batch_destroy(void *batch);
+
 #ifdef __cplusplus
 }
 #endif

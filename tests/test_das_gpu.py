@@ -217,3 +217,69 @@ def test_split_launch_small_image(gpu, oracle, prec):
     for _ in range(2):
         Im = plan.forward(Yt, 0.02, 0.05, 1500.0).double().cpu().numpy()
         assert rel_l2(Im, ref) <= (1e-13 if prec == "f64" else 2e-6)
+
+
+def _uneven_problems(method, rng, n, dtype):
+    """n problems of one method with different arrays, grids and A-scan lengths."""
+    probs = []
+    for i in range(n):
+        L = int(rng.integers(3, 12))
+        K = int(rng.integers(300, 900))
+        No = int(rng.integers(1, 700))
+        x = np.linspace(-3.0 - i % 3, 3.0 + i % 2, L)
+        Gt = np.c_[x, np.zeros(L), np.zeros(L)]
+        Gr = Gt.copy() if method == "tfm" else np.c_[np.zeros(L), x, np.zeros(L)]
+        n_ascans = L if method == "saft" else L * L
+        Ro = np.c_[rng.uniform(-5, 5, No), rng.uniform(-4, 4, No), rng.uniform(0.5, 8, No)]
+        Yh = rng.standard_normal((n_ascans, K))
+        probs.append(dict(K=K, Ro=Ro, Gt=Gt, Gr=None if method == "saft" else Gr,
+                          Yh=Yh, Y=torch.tensor(Yh, dtype=dtype, device="cuda"),
+                          dt=0.02, delay=0.05 * (i % 3), cp=1500.0))
+    return probs
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32_ref"])
+def test_uneven_batch_equals_single_calls(gpu, method, prec):
+    """One persistent launch over uneven problems gives each problem's single-call
+    result: indices bit for bit (ramp data), values to rounding (random data)."""
+    from dream_amd.torch_ops import DasBatch, DasPlan
+    rng = np.random.default_rng(11)
+    dtype = torch.float64 if prec == "f64" else torch.float32
+    probs = _uneven_problems(method, rng, 7, dtype)
+    batch = DasBatch(method, probs, precision=prec)
+    outs = [o.clone() for o in batch.run(use_graph=False)]
+    for p, o in zip(probs, outs):
+        plan = DasPlan(method, p["K"], p["Ro"], p["Gt"], p["Gr"], precision=prec)
+        ref = plan.forward(p["Y"], p["dt"], p["delay"], p["cp"]).double().cpu().numpy()
+        tol = 1e-13 if prec == "f64" else 2e-6
+        assert rel_l2(o.double().cpu().numpy(), ref) <= tol
+    # Ramp data Y[a, k] = k: every sum is an exact integer in any order.
+    for p in probs:
+        p["Y"].copy_(torch.arange(p["K"], dtype=dtype, device="cuda").expand_as(p["Y"]))
+    ramp = DasBatch(method, probs, precision=prec)
+    got = [o.clone() for o in ramp.run(use_graph=False)]
+    for p, o in zip(probs, got):
+        plan = DasPlan(method, p["K"], p["Ro"], p["Gt"], p["Gr"], precision=prec)
+        ref = plan.forward(p["Y"], p["dt"], p["delay"], p["cp"])
+        assert torch.equal(o, ref)
+
+
+def test_uneven_batch_graph_replays_and_oracle(gpu, oracle):
+    from dream_amd.torch_ops import DasBatch
+    rng = np.random.default_rng(12)
+    probs = _uneven_problems("tfm", rng, 9, torch.float64)
+    batch = DasBatch("tfm", probs, precision="f64")
+    first = [o.clone() for o in batch.run(use_graph=True)]
+    for _ in range(3):
+        torch.cuda.synchronize()
+        scratch = torch.randn(1 << 16, device="cuda")      # eager work between replays
+        scratch.mul_(2.0)
+        outs = batch.run(use_graph=True)
+        torch.cuda.synchronize()
+        for a, b in zip(first, outs):
+            assert torch.equal(a, b)
+    for p, o in zip(probs[:3], first[:3]):
+        ref, _ = oracle.das(p["Yh"].T.copy(), p["Gt"], p["Gr"], p["Ro"], p["dt"], p["delay"],
+                            p["cp"], "tfm", "ignore", "f64", nthreads=4)
+        assert np.max(np.abs(o.cpu().numpy() - ref)) <= 1e-12 * max(np.abs(ref).max(), 1.0)

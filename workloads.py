@@ -128,6 +128,51 @@ def c3(seed=3, n_problems=32):
     return probs
 
 
+def sir_small_uneven_batch(seed=13, n_problems=128):
+    """128 small dream_arr_rect problems of uneven size: 16-256 points, 4-16
+    elements of 0.2-1.0 x 2-6 mm, nt 200-800, the rest as in c3()."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for _ in range(n_problems):
+        L = int(rng.integers(4, 17))
+        a = rng.uniform(0.2, 1.0)
+        b = rng.uniform(2.0, 6.0)
+        G = linear_array(L, a + 0.1)
+        No = int(rng.integers(16, 257))
+        nt = int(rng.integers(200, 801))
+        half = G[-1, 0] + a / 2
+        Ro = np.c_[rng.uniform(-half, half, No), np.zeros(No), rng.uniform(5.0, 40.0, No)]
+        cp, dt = 1500.0, 0.02
+        fx, fz = rng.uniform(-half, half), rng.uniform(5.0, 40.0)
+        r = np.hypot(G[:, 0] - fx, fz)
+        focal = (r.max() - r) * 1000.0 / cp
+        dmin = np.sqrt(np.maximum(np.abs(Ro[:, 0:1] - G[:, 0][None, :]) - a / 2, 0.0) ** 2
+                       + Ro[:, 2:3] ** 2).min(axis=1)
+        delay = dmin * 1000.0 / cp - focal.max() - 0.5
+        probs.append(dict(Ro=Ro, a=a, b=b, dx=0.05, dy=0.05, dt=dt, nt=nt, delay=delay, v=1.0,
+                          cp=cp, G=G, foc_met="ud", focal=focal, apod_met="hann",
+                          err_level="ignore"))
+    return probs
+
+
+def das_uneven_batch(seed=14, n_problems=96, device="cpu", dtype=torch.float32):
+    """96 small TFM problems of uneven size: linear arrays of 8-48 elements at
+    0.5 mm pitch, A-scans of 512-2048 samples at 50 MHz, square pixel grids of
+    16-80 pixels a side at 0.125 mm; random data (throughput only)."""
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=device).manual_seed(seed)
+    probs = []
+    for _ in range(n_problems):
+        L = int(rng.integers(8, 49))
+        K = int(rng.integers(512, 2049))
+        n = int(rng.integers(16, 81))
+        G = linear_array(L, 0.5)
+        Ro = _grid_xz(n, n, 0.125, -n * 0.0625, 10.0)
+        Y = torch.randn((L * L, K), generator=gen, device=device, dtype=dtype)
+        probs.append(dict(K=K, Ro=Ro, Gt=G, Gr=G, Y=Y, dt=0.02, delay=0.0, cp=1500.0))
+    return probs
+
+
 def c4(seed=4, device="cpu", dtype=torch.float32):
     rng = np.random.default_rng(seed)
     gen = torch.Generator(device=device).manual_seed(seed)
