@@ -26,6 +26,7 @@ DAS_TYPE = {"saft": 0, "tfm": 1, "rca": 2, "rca_coltx": 2, "rca_rowtx": 3}
 # Every symbol include/dream_hip.h declares.
 EXPORTS = (
     "dream_hip_last_error", "dream_hip_version", "dream_hip_device_count",
+    "dream_hip_set_autotune",
     "dream_hip_abort", "dream_hip_set_running", "dream_hip_is_running",
     "dream_hip_arr_rect", "dream_hip_rect", "dream_hip_arr_rect_dev",
     "dream_hip_rect_indices", "dream_hip_sir_plan_create", "dream_hip_sir_plan_run",
@@ -83,6 +84,8 @@ def _declare(L):
     L.dream_hip_version.argtypes = []
     L.dream_hip_device_count.restype = _i32
     L.dream_hip_device_count.argtypes = []
+    L.dream_hip_set_autotune.restype = None
+    L.dream_hip_set_autotune.argtypes = [_i32]
     L.dream_hip_abort.restype = None
     L.dream_hip_abort.argtypes = [_i32]
     L.dream_hip_set_running.restype = None

@@ -11,6 +11,7 @@
 
 #include "das.hpp"
 #include "sir.hpp"
+#include "tune.hpp"
 
 namespace dream {
 
@@ -132,6 +133,8 @@ int dream_hip_device_count(void)
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
 }
+
+void dream_hip_set_autotune(int on) { set_autotune(on != 0); }
 
 void dream_hip_abort(int) { g_running.store(false); }
 void dream_hip_set_running(void) { g_running.store(true); }

@@ -30,12 +30,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <utility>
 #include <vector>
 
 #include "das.hpp"
+#include "tune.hpp"
 
 namespace dream {
 
@@ -973,10 +975,32 @@ FwdShape forward_shape(int type, int64_t No, int64_t n_frames, int64_t Lt, int64
   return sh;
 }
 
-template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
-void fwd_launch(const DasArgs &a, hipStream_t s)
+// Candidate launch shapes for the autotuner; the heuristic shape first.
+std::vector<FwdShape> forward_candidates(int type, int64_t No, int64_t n_frames, int64_t Lt,
+                                         int64_t Lr)
 {
-  FwdShape sh = forward_shape(TYPE, a.No, a.n_frames, a.Lt, a.Lr);
+  const FwdShape h = forward_shape(type, No, n_frames, Lt, Lr);
+  const int64_t J = type == DAS_SAFT ? (Lt + kSB - 1) / kSB
+                                     : ((Lr + kRC - 1) / kRC) * ((Lt + kTB - 1) / kTB);
+  std::vector<FwdShape> c{h};
+  const int splits[3] = {1, h.split, 2 * h.split};
+  for (int w : {2, 4, 8})
+    for (int sp : splits) {
+      if (w > std::max<int64_t>(J, 1) || sp > 64) continue;
+      if ((int64_t) w * sp > std::max<int64_t>(J, 1) * 2) continue;   // most waves idle
+      bool dup = false;
+      for (const FwdShape &x : c) dup |= (x.waves == w && x.split == sp);
+      if (!dup) c.push_back(FwdShape{w, sp});
+    }
+  return c;
+}
+
+template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
+void fwd_launch(const DasArgs &a, hipStream_t s);
+
+template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
+void fwd_launch_shape(const DasArgs &a, FwdShape sh, hipStream_t s)
+{
   if (sh.split > 1 && (size_t) sh.split * a.n_frames * a.No * sizeof(DT) > a.scratch_bytes)
     sh.split = 1;
   dim3 grid((unsigned) ((a.No + 63) / 64), (unsigned) sh.split, (unsigned) a.n_frames);
@@ -993,6 +1017,26 @@ void fwd_launch(const DasArgs &a, hipStream_t s)
                        s, (const DT *) a.scratch, sh.split, a.No, a.n_frames, a.im_stride,
                        (DT *) a.Im);
   }
+}
+
+// The heuristic shape, or the autotuner's pick for this shape class (not
+// when the launch also collects the out-of-bounds status).
+template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
+void fwd_launch(const DasArgs &a, hipStream_t s)
+{
+  const std::vector<FwdShape> cand = forward_candidates(TYPE, a.No, a.n_frames, a.Lt, a.Lr);
+  int pick = 0;
+  if (!a.status && cand.size() > 1) {
+    char key[192];
+    std::snprintf(key, sizeof key, "das_fwd|%d|%d|%d|%d|%d|%lld|%d|%d|%lld|%lld", TYPE,
+                  (int) sizeof(DT), (int) sizeof(GT), INTERP, (int) APOD, (long long) a.No, a.Lt,
+                  a.Lr, (long long) a.K, (long long) a.n_frames);
+    const int t = autotune(key, (int) cand.size(),
+                           [&](int i, hipStream_t st) { fwd_launch_shape<DT, GT, TYPE, INTERP, APOD>(a, cand[(size_t) i], st); },
+                           s);
+    if (t >= 0) pick = t;
+  }
+  fwd_launch_shape<DT, GT, TYPE, INTERP, APOD>(a, cand[(size_t) pick], s);
 }
 
 template <typename DT, typename GT, int TYPE>
@@ -1125,8 +1169,10 @@ size_t geom_size(int prec) { return prec == DREAM_HIP_F32_REF ? 4 : 8; }
 size_t das_forward_scratch_bytes(int type, int prec, int64_t No, int64_t n_frames, int64_t Lt,
                                  int64_t Lr)
 {
-  const FwdShape sh = forward_shape(Lr == 0 ? DAS_SAFT : type, No, n_frames, Lt, Lr);
-  return sh.split > 1 ? (size_t) sh.split * (size_t) n_frames * (size_t) No * elem_size(prec) : 0;
+  int split = 1;
+  for (const FwdShape &sh : forward_candidates(Lr == 0 ? DAS_SAFT : type, No, n_frames, Lt, Lr))
+    split = std::max(split, sh.split);
+  return split > 1 ? (size_t) split * (size_t) n_frames * (size_t) No * elem_size(prec) : 0;
 }
 
 template <typename GT>
@@ -1552,9 +1598,13 @@ DasBatch::DasBatch(const DreamDasProblem *probs, int32_t n, int type, int precis
     DREAM_HIP_CHECK(hipMemcpy(d_begin_, begin.data(), begin.size() * sizeof(int64_t),
                               hipMemcpyHostToDevice));
   }
-  // Up to 8 waves per SIMD, never more waves than items.
-  const int64_t waves = std::min<int64_t>((int64_t) cu_count() * 32, n_items_);
-  grid_ = (int) std::max<int64_t>(1, (waves + 3) / 4);
+  // Candidate grids: 2, 4 or 8 waves per SIMD, never more waves than items;
+  // the first is the default when the autotuner does not run.
+  for (int per_simd : {8, 4, 2}) {
+    const int64_t waves = std::min<int64_t>((int64_t) cu_count() * 4 * per_simd, n_items_);
+    const int g = (int) std::max<int64_t>(1, (waves + 3) / 4);
+    if (std::find(grids_.begin(), grids_.end(), g) == grids_.end()) grids_.push_back(g);
+  }
 }
 
 DasBatch::~DasBatch()
@@ -1566,24 +1616,39 @@ DasBatch::~DasBatch()
   (void) hipFree(d_counter_);
 }
 
-void DasBatch::launch(hipStream_t s)
+void DasBatch::launch_grid(int grid, hipStream_t s)
 {
   if (n_items_ == 0) return;
   if (prec_ == DREAM_HIP_F64)
     batch_dispatch_type<double, double>(type_, interp_, apod_, d_args_, d_begin_, n_probs_,
-                                        n_items_, d_counter_, grid_, s);
+                                        n_items_, d_counter_, grid, s);
   else if (prec_ == DREAM_HIP_F32)
     batch_dispatch_type<float, double>(type_, interp_, apod_, d_args_, d_begin_, n_probs_,
-                                       n_items_, d_counter_, grid_, s);
+                                       n_items_, d_counter_, grid, s);
   else
     batch_dispatch_type<float, float>(type_, interp_, apod_, d_args_, d_begin_, n_probs_,
-                                      n_items_, d_counter_, grid_, s);
+                                      n_items_, d_counter_, grid, s);
   DREAM_HIP_CHECK(hipGetLastError());
+}
+
+void DasBatch::launch(hipStream_t s) { launch_grid(grids_[(size_t) pick_], s); }
+
+void DasBatch::choose(hipStream_t s)
+{
+  if (pick_ >= 0) return;
+  char key[160];
+  std::snprintf(key, sizeof key, "das_batch|%d|%d|%d|%d|%d|%lld", type_, prec_, interp_,
+                (int) apod_, n_probs_, (long long) n_items_);
+  const int t = n_items_ ? autotune(key, (int) grids_.size(),
+                                    [&](int i, hipStream_t st) { launch_grid(grids_[(size_t) i], st); }, s)
+                         : -1;
+  pick_ = t >= 0 ? t : 0;
 }
 
 void DasBatch::run(hipStream_t s, bool use_graph)
 {
   if (cancelled()) throw Error(DREAM_HIP_ECANCELLED, "das batch cancelled");
+  choose(s);
   if (!use_graph) {
     launch(s);
     return;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 #include "common.hpp"
 
@@ -101,12 +102,15 @@ public:
   ~DasBatch();
   void run(hipStream_t s, bool use_graph);
 private:
+  void choose(hipStream_t s);              // launch grid: autotuned or default
+  void launch_grid(int grid, hipStream_t s);
   void launch(hipStream_t s);
   int type_, prec_, interp_;
   bool apod_ = false;
   int32_t n_probs_ = 0;
   int64_t n_items_ = 0;
-  int grid_ = 1;
+  std::vector<int> grids_;                 // candidate grids, default first
+  int pick_ = -1;
   DasArgs *d_args_ = nullptr;
   int64_t *d_begin_ = nullptr;
   void *d_delay_ = nullptr;

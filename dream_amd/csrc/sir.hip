@@ -34,10 +34,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 
 #include "sir.hpp"
+#include "tune.hpp"
 
 namespace dream {
 
@@ -567,6 +569,18 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
   const int wg_per_cu = lds_bytes_ ? (int) std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / std::max<size_t>(lds_bytes_, 1))) : 8;
   persistent_ = P > 1 && n_items_ > (int64_t) cus * wg_per_cu * 4;
   grid_ = persistent_ ? cus * wg_per_cu : (int) std::min<int64_t>(n_items_, 1ll << 31);
+  // Launch shapes for the autotuner (the heuristic first): for batches, one
+  // work-group per item and persistent grids of 1, 2 or wg_per_cu
+  // work-groups per CU.
+  shapes_.push_back(SirShape{persistent_, grid_});
+  if (P > 1 && n_items_ < (1ll << 31)) {
+    auto add = [&](bool pers, int grid) {
+      for (const SirShape &x : shapes_) if (x.persistent == pers && x.grid == grid) return;
+      shapes_.push_back(SirShape{pers, grid});
+    };
+    add(false, (int) n_items_);
+    for (int k : {wg_per_cu, 2, 1}) add(true, cus * k);
+  }
 }
 
 SirPlan::~SirPlan()
@@ -578,7 +592,23 @@ SirPlan::~SirPlan()
   (void) hipFree(d_counter_);
 }
 
-void SirPlan::launch(hipStream_t s)
+void SirPlan::choose(hipStream_t s)
+{
+  if (pick_ >= 0) return;
+  int64_t total_no = 0;
+  for (const SirDev &d : host_) total_no += d.No;
+  char key[160];
+  std::snprintf(key, sizeof key, "sir|%d|%zu|%lld|%lld|%zu", out_dtype_, host_.size(),
+                (long long) n_items_, (long long) total_no, lds_bytes_);
+  const int t = n_items_ ? autotune(key, (int) shapes_.size(),
+                                    [&](int i, hipStream_t st) { launch_shape(shapes_[(size_t) i], st); }, s)
+                         : -1;
+  pick_ = t >= 0 ? t : 0;
+}
+
+void SirPlan::launch(hipStream_t s) { launch_shape(shapes_[(size_t) std::max(pick_, 0)], s); }
+
+void SirPlan::launch_shape(const SirShape &sh, hipStream_t s)
 {
   launch_fill_zero(d_status_, host_.size() * sizeof(Status), s);
   for (size_t i = 0; i < zero_len_.size(); i++)
@@ -587,12 +617,12 @@ void SirPlan::launch(hipStream_t s)
       launch_fill_zero(zero_ptr_[i], (size_t) zero_len_[i] * esz, s);
     }
   if (n_items_ == 0) return;
-  unsigned long long *ctr = persistent_ ? d_counter_ : nullptr;
+  unsigned long long *ctr = sh.persistent ? d_counter_ : nullptr;
   if (out_dtype_ == DREAM_HIP_F64) {
-    hipLaunchKernelGGL(sir_kernel<double>, dim3(grid_), dim3(kThreads), lds_bytes_, s,
+    hipLaunchKernelGGL(sir_kernel<double>, dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
                        d_probs_, (int) host_.size(), n_items_, ctr);
   } else {
-    hipLaunchKernelGGL(sir_kernel<float>, dim3(grid_), dim3(kThreads), lds_bytes_, s,
+    hipLaunchKernelGGL(sir_kernel<float>, dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
                        d_probs_, (int) host_.size(), n_items_, ctr);
   }
   DREAM_HIP_CHECK(hipGetLastError());
@@ -603,6 +633,7 @@ void SirPlan::run(hipStream_t s, bool use_graph)
   if (cancelled()) throw Error(DREAM_HIP_ECANCELLED, "SIR cancelled");
   last_stream_ = s;
   ran_ = true;
+  choose(s);
   if (!use_graph) {
     launch(s);
     return;

@@ -62,7 +62,15 @@ public:
   int64_t n_items() const { return n_items_; }
 
 private:
+  struct SirShape {
+    bool persistent;   // grid pulls items from the counter
+    int grid;
+  };
+  void choose(hipStream_t s);               // launch shape: autotuned or heuristic
   void launch(hipStream_t s);
+  void launch_shape(const SirShape &sh, hipStream_t s);
+  std::vector<SirShape> shapes_;            // candidates, heuristic first
+  int pick_ = -1;
   int out_dtype_;
   std::vector<SirDev> host_;
   std::vector<int> levels_;

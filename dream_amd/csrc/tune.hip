@@ -1,0 +1,136 @@
+// tune.hip -- launch-shape autotuner (see tune.hpp).
+
+#include "tune.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <vector>
+
+namespace dream {
+
+namespace {
+
+std::mutex g_mu;
+std::map<std::string, int> g_cache;
+bool g_loaded = false;
+int g_enabled = -1;          // -1: not yet read from the environment
+
+std::string arch_name()
+{
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return "unknown";
+  std::string name = prop.gcnArchName;
+  const size_t colon = name.find(':');
+  return colon == std::string::npos ? name : name.substr(0, colon);
+}
+
+const char *cache_file() { return std::getenv("DREAM_HIP_TUNE_CACHE"); }
+
+void load_file_locked()
+{
+  if (g_loaded) return;
+  g_loaded = true;
+  const char *path = cache_file();
+  if (!path) return;
+  std::ifstream in(path);
+  std::string line;
+  while (std::getline(in, line)) {
+    const size_t sp = line.rfind(' ');
+    if (sp == std::string::npos) continue;
+    g_cache[line.substr(0, sp)] = std::atoi(line.c_str() + sp + 1);
+  }
+}
+
+void append_file_locked(const std::string &key, int best)
+{
+  const char *path = cache_file();
+  if (!path) return;
+  std::ofstream out(path, std::ios::app);
+  out << key << ' ' << best << '\n';
+}
+
+} // namespace
+
+bool autotune_enabled()
+{
+  std::lock_guard<std::mutex> lock(g_mu);
+  if (g_enabled < 0) {
+    const char *e = std::getenv("DREAM_HIP_AUTOTUNE");
+    g_enabled = (e && e[0] == '0') ? 0 : 1;
+  }
+  return g_enabled != 0;
+}
+
+void set_autotune(bool on)
+{
+  std::lock_guard<std::mutex> lock(g_mu);
+  g_enabled = on ? 1 : 0;
+}
+
+int autotune_cache_size()
+{
+  std::lock_guard<std::mutex> lock(g_mu);
+  return (int) g_cache.size();
+}
+
+void autotune_clear()
+{
+  std::lock_guard<std::mutex> lock(g_mu);
+  g_cache.clear();
+}
+
+int autotune(const std::string &key, int n, const std::function<void(int, hipStream_t)> &run,
+             hipStream_t s)
+{
+  if (n <= 1) return n == 1 ? 0 : -1;
+  if (!autotune_enabled()) return -1;
+  const std::string full = arch_name() + "|" + key;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    load_file_locked();
+    auto it = g_cache.find(full);
+    if (it != g_cache.end() && it->second >= 0 && it->second < n) return it->second;
+  }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return -1;
+
+  constexpr int kReps = 5;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  DREAM_HIP_CHECK(hipEventCreate(&e0));
+  DREAM_HIP_CHECK(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 0.0f;
+  try {
+    for (int i = 0; i < n; i++) {
+      run(i, s);                                  // warm-up
+      DREAM_HIP_CHECK(hipEventRecord(e0, s));
+      for (int r = 0; r < kReps; r++) run(i, s);
+      DREAM_HIP_CHECK(hipEventRecord(e1, s));
+      DREAM_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      DREAM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (i == 0 || ms < best_ms) {
+        best = i;
+        best_ms = ms;
+      }
+    }
+  } catch (...) {
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
+    throw;
+  }
+  (void) hipEventDestroy(e0);
+  (void) hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lock(g_mu);
+  g_cache[full] = best;
+  append_file_locked(full, best);
+  return best;
+}
+
+} // namespace dream

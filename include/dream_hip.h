@@ -73,6 +73,16 @@ const char *dream_hip_version(void);
 /* Number of visible HIP devices (0 when there is none); never fails. */
 int dream_hip_device_count(void);
 
+/* Launch-shape autotuning (on by default; DREAM_HIP_AUTOTUNE=0 turns it
+ * off at start-up).  When on, the first launch of a DAS forward, DAS batch
+ * or SIR batch shape class times the candidate launch shapes (work-group
+ * size, split, persistent grid) on the caller's stream and keeps the
+ * fastest, cached per process by (device architecture, shape class) and,
+ * when DREAM_HIP_TUNE_CACHE names a file, in that file.  Results are the
+ * same for every shape up to the summation order.  No counterpart in the
+ * reference. */
+void dream_hip_set_autotune(int on);
+
 /* Cooperative cancellation: dream_hip_abort() makes running calls stop
  * between work chunks and return DREAM_HIP_ECANCELLED; it is safe to call
  * from a signal handler.  Mirrors ArrRect::abort / DAS<T>::abort /

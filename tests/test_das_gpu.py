@@ -283,3 +283,27 @@ def test_uneven_batch_graph_replays_and_oracle(gpu, oracle):
         ref, _ = oracle.das(p["Yh"].T.copy(), p["Gt"], p["Gr"], p["Ro"], p["dt"], p["delay"],
                             p["cp"], "tfm", "ignore", "f64", nthreads=4)
         assert np.max(np.abs(o.cpu().numpy() - ref)) <= 1e-12 * max(np.abs(ref).max(), 1.0)
+
+
+def test_autotuned_shapes_give_the_same_image(gpu):
+    """The forward launch shape picked by the autotuner (waves per work-group,
+    split) changes only the summation order: ramp data give identical images
+    with tuning on and off, for a small image that has split candidates."""
+    from dream_amd import _lib
+    from dream_amd.torch_ops import DasPlan
+    rng = np.random.default_rng(21)
+    L, K, No = 48, 900, 300
+    x = np.linspace(-6.0, 6.0, L)
+    G = np.c_[x, np.zeros(L), np.zeros(L)]
+    Ro = np.c_[rng.uniform(-6, 6, No), np.zeros(No), rng.uniform(1, 8, No)]
+    Y = torch.arange(K, dtype=torch.float32, device="cuda").expand(L * L, K).contiguous()
+    try:
+        _lib.lib().dream_hip_set_autotune(0)
+        off = DasPlan("tfm", K, Ro, G, G, precision="f32").forward(Y, 0.02, 0.05, 1500.0)
+        _lib.lib().dream_hip_set_autotune(1)
+        plan = DasPlan("tfm", K, Ro, G, G, precision="f32")
+        first = plan.forward(Y, 0.02, 0.05, 1500.0).clone()     # tunes
+        again = plan.forward(Y, 0.02, 0.05, 1500.0)             # reuses the pick
+        assert torch.equal(first, off) and torch.equal(again, off)
+    finally:
+        _lib.lib().dream_hip_set_autotune(1)

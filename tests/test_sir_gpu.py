@@ -204,3 +204,32 @@ def test_batch_plan_uneven_with_graph_replays(gpu, oracle):
             assert np.max(np.abs(H.cpu().numpy().T - ref)) <= 1e-12 * np.abs(ref).max(), rep
             assert o == oob
         torch.cuda.synchronize()
+
+
+def test_autotuned_batch_equals_untuned(gpu):
+    """Every SIR batch launch shape (one work-group per item, persistent
+    grids) computes each item the same way: tuned and untuned runs agree."""
+    import torch
+
+    import workloads
+    from dream_amd import _lib
+    from dream_amd.torch_ops import SirBatch
+    probs = workloads.sir_small_uneven_batch(n_problems=12)
+    dev = []
+    for p in probs:
+        q = dict(p)
+        q["Ro"] = torch.tensor(p["Ro"].T.copy(), device="cuda")
+        q["delay"] = torch.tensor(p["delay"], device="cuda")
+        dev.append(q)
+    try:
+        _lib.lib().dream_hip_set_autotune(0)
+        ref = [h.clone() for h in SirBatch(dev, out_dtype=torch.float64).run(use_graph=False)]
+        _lib.lib().dream_hip_set_autotune(1)
+        tuned = SirBatch(dev, out_dtype=torch.float64)
+        for _ in range(2):
+            got = tuned.run(use_graph=True)
+            torch.cuda.synchronize()
+            for a, b in zip(got, ref):
+                assert torch.allclose(a, b, rtol=1e-12, atol=1e-12 * float(b.abs().max()))
+    finally:
+        _lib.lib().dream_hip_set_autotune(1)
