@@ -226,6 +226,25 @@ def bench_batches(steps, warmup):
     return out
 
 
+def bench_conv(steps, warmup):
+    """Data synthesis by convolution (fftconv_p / sum_fftconv path): 4096
+    columns (64 x 64 A-scans) of 2000 samples convolved with one 300-sample
+    pulse, FP64 and FP32, as multiply-adds per second."""
+    import torch
+
+    from dream_amd.torch_ops import conv_sum
+    out = {}
+    for name, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        g = torch.Generator(device="cuda").manual_seed(7)
+        A = torch.randn((4096, 2000), generator=g, device="cuda", dtype=dt)
+        b = torch.randn(300, generator=g, device="cuda", dtype=dt)
+        y = torch.empty((4096, 2299), device="cuda", dtype=dt)
+        ms = time_launches(lambda: conv_sum(A, b, out=y), steps, warmup)
+        out[f"conv_4096x2000_by_300_{name}"] = dict(ms=round(ms, 4),
+                                                   macs_per_s=4096 * 2000 * 300 / (ms * 1e-3))
+    return out
+
+
 def bench_das_diff(steps, warmup):
     """C4 SAFT (linear, apodized) forward and adjoint; C5 RCA 8 frames forward and
     apodization adjoint."""
@@ -313,10 +332,11 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: headline kernel only, plus a 256 MiB "
                          "device-to-device copy used to calibrate the HBM byte counters")
-    ap.add_argument("--part", choices=("sir", "c4", "c5", "batch"),
+    ap.add_argument("--part", choices=("sir", "c4", "c5", "batch", "conv"),
                     help="profile one of the other paths alone (short run, no JSON line): "
                          "sir = C1 + C3 SIR, c4 = SAFT forward/adjoint, c5 = RCA frames, "
-                         "batch = uneven batches of small DAS and SIR problems")
+                         "batch = uneven batches of small DAS and SIR problems, "
+                         "conv = data-synthesis convolution")
     args = ap.parse_args()
     if args.profile:
         args.no_cpu = True
@@ -331,7 +351,7 @@ def main():
     _lib.require_device()
     if args.part:
         part = {"sir": bench_sir, "c4": bench_das_diff_c4, "c5": bench_das_diff_c5,
-                "batch": bench_batches}[args.part]
+                "batch": bench_batches, "conv": bench_conv}[args.part]
         log(json.dumps(part(args.steps, args.warmup)))
         return
 
@@ -347,6 +367,7 @@ def main():
     if not args.no_extra:
         extra.update(bench_sir(args.steps, args.warmup))
         extra.update(bench_batches(args.steps, args.warmup))
+        extra.update(bench_conv(args.steps, args.warmup))
         extra.update(bench_das_diff(args.steps, args.warmup))
     line = {
         "metric": METRIC,

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libdream_hip.so")
 OK = 0
 F64, F32_REF, F32 = 0, 1, 2
 NEAREST, LINEAR = 0, 1
+CONV_EQU, CONV_SUM, CONV_NEG = 0, 1, 2
 
 # Enum numbering of the reference (include/dream.h:42-102).
 ERR_LEVEL = {"ignore": 1, "warn": 2, "stop": 3}
@@ -35,6 +36,7 @@ EXPORTS = (
     "dream_hip_das_set_pixel_order", "dream_hip_pixel_order",
     "dream_hip_das", "dream_hip_das_dev", "dream_hip_das_adjoint_dev",
     "dream_hip_das_batch_create", "dream_hip_das_batch_run", "dream_hip_das_batch_destroy",
+    "dream_hip_fftconv_p", "dream_hip_sum_fftconv", "dream_hip_conv_sum_dev",
 )
 
 _i32, _i64, _dbl = ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -128,6 +130,13 @@ def _declare(L):
     L.dream_hip_das_dev.restype = _i32
     L.dream_hip_das_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _dbl, _i32, _vp, _dbl, _vp, _vp,
                                     _i32, _i64, _i64, _i64, _vp, _i32, _ip, _vp]
+    L.dream_hip_fftconv_p.restype = _i32
+    L.dream_hip_fftconv_p.argtypes = [_vp, _vp, _i64, _i64, _vp, _i64, _i64, _i32]
+    L.dream_hip_sum_fftconv.restype = _i32
+    L.dream_hip_sum_fftconv.argtypes = [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _i32]
+    L.dream_hip_conv_sum_dev.restype = _i32
+    L.dream_hip_conv_sum_dev.argtypes = [_vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, _i64, _i64,
+                                         _i64, _i32, _i32, _vp]
     L.dream_hip_das_batch_create.restype = _i32
     L.dream_hip_das_batch_create.argtypes = [ctypes.POINTER(DreamDasProblem), ctypes.c_int32,
                                              _i32, _i32, _i32, ctypes.POINTER(_vp)]

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "conv.hpp"
 #include "das.hpp"
 #include "sir.hpp"
 #include "tune.hpp"
@@ -358,6 +359,75 @@ int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const
                                                   im_frame_stride, g, gY, g_apod_t, g_apod_r,
                                                   g_cp, s);
     return DREAM_HIP_OK;
+  });
+}
+
+// ------------------------------------------------------------------ convolution
+
+int dream_hip_conv_sum_dev(void *Y, const void *A, int64_t M_A, int64_t N, int64_t L,
+                           int64_t a_col_stride, int64_t a_layer_stride, const void *B,
+                           int64_t M_B, int64_t b_col_stride, int64_t b_layer_stride, int mode,
+                           int dtype, void *stream)
+{
+  return guarded([&]() {
+    require(N == 0 || (Y && A && B), "conv: null device pointer");
+    require(dtype == DREAM_HIP_F64 || dtype == DREAM_HIP_F32, "conv: dtype must be F64 or F32");
+    hipStream_t s = (hipStream_t) stream;
+    if (dtype == DREAM_HIP_F64)
+      launch_conv_sum<double>((double *) Y, (const double *) A, M_A, N, L, a_col_stride,
+                              a_layer_stride, (const double *) B, M_B, b_col_stride,
+                              b_layer_stride, mode, s);
+    else
+      launch_conv_sum<float>((float *) Y, (const float *) A, M_A, N, L, a_col_stride,
+                             a_layer_stride, (const float *) B, M_B, b_col_stride,
+                             b_layer_stride, mode, s);
+    return DREAM_HIP_OK;
+  });
+}
+
+namespace {
+// Host-pointer convolution: copy in, convolve, copy out (synchronous).
+int host_conv_sum(double *Y, const double *A, int64_t MA, int64_t N, int64_t L, int64_t a_col,
+                  int64_t a_layer, int64_t a_total, const double *B, int64_t MB, int64_t b_col,
+                  int64_t b_layer, int64_t b_total, int mode)
+{
+  require(MA >= 1 && MB >= 1 && N >= 0 && L >= 1, "conv: bad sizes");
+  require(mode == CONV_EQU || mode == CONV_SUM || mode == CONV_NEG, "conv: unknown mode");
+  const int64_t MY = MA + MB - 1;
+  if (N == 0) return DREAM_HIP_OK;
+  require(Y && A && B, "conv: null argument");
+  hipStream_t s = lib_stream();
+  DevBuf dA(A, (size_t) a_total * 8, s), dB(B, (size_t) b_total * 8, s);
+  DevBuf dY = mode == CONV_EQU ? DevBuf((size_t) (MY * N) * 8) : DevBuf(Y, (size_t) (MY * N) * 8, s);
+  launch_conv_sum<double>((double *) dY.p, (const double *) dA.p, MA, N, L, a_col, a_layer,
+                          (const double *) dB.p, MB, b_col, b_layer, mode, s);
+  DREAM_HIP_CHECK(hipMemcpyAsync(Y, dY.p, (size_t) (MY * N) * 8, hipMemcpyDeviceToHost, s));
+  DREAM_HIP_CHECK(hipStreamSynchronize(s));
+  return DREAM_HIP_OK;
+}
+} // namespace
+
+int dream_hip_fftconv_p(double *Y, const double *A, int64_t A_M, int64_t A_N, const double *B,
+                        int64_t B_M, int64_t B_N, int mode)
+{
+  return guarded([&]() {
+    require(A_M >= 1 && A_N >= 0 && B_M >= 1 && B_N >= 1, "fftconv_p: bad sizes");
+    // A vector B is one filter for every column (extra_src/fftconv_p.cc:170-173).
+    const bool vec = B_M == 1 || B_N == 1;
+    require(vec || B_N == A_N,
+            "fftconv_p: B must be a vector or a matrix with the same number of columns as A");
+    const int64_t MB = vec ? B_M * B_N : B_M;
+    return host_conv_sum(Y, A, A_M, A_N, 1, A_M, 0, A_M * A_N, B, MB, vec ? 0 : B_M, 0,
+                         vec ? MB : B_M * B_N, mode);
+  });
+}
+
+int dream_hip_sum_fftconv(double *Y, const double *H, int64_t M, int64_t N, int64_t L,
+                          const double *U, int64_t K, int mode)
+{
+  return guarded([&]() {
+    require(M >= 1 && N >= 0 && L >= 1 && K >= 1, "sum_fftconv: bad sizes");
+    return host_conv_sum(Y, H, M, N, L, M, M * N, M * N * L, U, K, 0, K, K * L, mode);
   });
 }
 

@@ -359,3 +359,40 @@ class DasBatch:
                 _lib._lib.dream_hip_das_batch_destroy(self.batch)
         except Exception:
             pass
+
+
+# ----------------------------------------------------------------------- convolution
+
+CONV_MODE = {"=": _lib.CONV_EQU, "+=": _lib.CONV_SUM, "-=": _lib.CONV_NEG}
+
+
+def conv_sum(A, B, out=None, mode="="):
+    """Column-wise linear convolution on device tensors, summed over layers.
+
+    A: (N, M_A) or (L, N, M_A) -- rows are signals (row-major views of the
+    reference's column-major M_A x N / M_A x N x L arrays); B: (M_B,) one
+    filter, (N, M_B) one per column, or (L, M_B) one per layer.  Returns or
+    updates out (N, M_A + M_B - 1): out[n] (mode) sum_l A[l, n] (*) B[...].
+    float64 or float32.
+    """
+    A3 = A if A.dim() == 3 else A.unsqueeze(0)
+    L, N, MA = A3.shape
+    assert A3.is_contiguous() and B.is_contiguous() and B.dtype == A.dtype
+    MB = B.shape[-1]
+    if B.dim() == 1:
+        b_col, b_layer = 0, 0
+    elif A.dim() == 2:
+        assert B.shape[0] == N, "B needs one row per column of A"
+        b_col, b_layer = MB, 0
+    else:
+        assert B.shape[0] == L, "B needs one row per layer of A"
+        b_col, b_layer = 0, MB
+    if out is None:
+        assert mode == "=", "in-place modes need `out`"
+        out = torch.empty((N, MA + MB - 1), dtype=A.dtype, device=A.device)
+    assert out.is_contiguous() and out.shape == (N, MA + MB - 1) and out.dtype == A.dtype
+    dt = _lib.F64 if A.dtype == torch.float64 else _lib.F32
+    _lib.check(_lib.lib().dream_hip_conv_sum_dev(_ptr(out), _ptr(A3), MA, N, L, MA, N * MA,
+                                                 _ptr(B), MB, b_col, b_layer, CONV_MODE[mode],
+                                                 dt, _stream()))
+    return out

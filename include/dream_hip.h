@@ -246,6 +246,40 @@ int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const
                               int64_t im_frame_stride, const void *g, void *gY,
                               void *g_apod_t, void *g_apod_r, void *g_cp, void *stream);
 
+/* ---------------------------------------------------------------- convolution */
+
+/* Convolution modes (ConvMode, include/dream.h:91-95). */
+#define DREAM_HIP_CONV_EQU 0         /* Y  = A (*) B */
+#define DREAM_HIP_CONV_SUM 1         /* Y += A (*) B */
+#define DREAM_HIP_CONV_NEG 2         /* Y -= A (*) B */
+
+/* FFTConvP::run (include/fftconv_p.h:39-44, extra_src/fftconv_p.cc:154-403):
+ * column-wise linear convolution Y(:,n) (mode) A(:,n) (*) B(:,n), or with
+ * the single filter B when B is a vector (B_M == 1 or B_N == 1).  HOST
+ * pointers, FP64, column-major; Y is (A_M + B_M - 1) x A_N (read for the
+ * '+=' and '-=' modes).  The reference multiplies zero-padded FFTs; this
+ * evaluates the same sum directly, so results agree to rounding. */
+int dream_hip_fftconv_p(double *Y, const double *A, int64_t A_M, int64_t A_N, const double *B,
+                        int64_t B_M, int64_t B_N, int mode);
+
+/* sum_fftconv (extra_src/oct_sum_fftconv.cc, add_fftconv in
+ * extra_src/fftconv.cc:119-209): Y(:,n) (mode) sum_l H(:,n,l) (*) U(:,l) for an
+ * M x N x L array H and a K x L matrix U.  HOST pointers, FP64; Y is
+ * (M + K - 1) x N.  The reference's gateway takes only DREAM_HIP_CONV_EQU
+ * (normal mode) and DREAM_HIP_CONV_SUM (in-place mode). */
+int dream_hip_sum_fftconv(double *Y, const double *H, int64_t M, int64_t N, int64_t L,
+                          const double *U, int64_t K, int mode);
+
+/* Device-pointer form of both: Y(:,n) (mode) sum_{l<L} A_l(:,n) (*) B_l(:,n)
+ * with A_l(:,n) at A + l*a_layer_stride + n*a_col_stride (M_A samples),
+ * B_l(:,n) at B + l*b_layer_stride + n*b_col_stride (M_B samples; stride 0
+ * shares a filter) and Y(:,n) at Y + n*(M_A + M_B - 1), element strides.
+ * dtype: DREAM_HIP_F64 or DREAM_HIP_F32.  Asynchronous on `stream`. */
+int dream_hip_conv_sum_dev(void *Y, const void *A, int64_t M_A, int64_t N, int64_t L,
+                           int64_t a_col_stride, int64_t a_layer_stride, const void *B,
+                           int64_t M_B, int64_t b_col_stride, int64_t b_layer_stride, int mode,
+                           int dtype, void *stream);
+
 /* Batched uneven DAS: many beamforming problems of one DAS type, precision
  * and sampling rule -- different arrays, pixel grids and A-scan lengths --
  * in one persistent forward launch.  No counterpart in the reference, whose

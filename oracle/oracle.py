@@ -281,3 +281,35 @@ def das_ext_adjoint(Y, Gt, Gr, Ro, dt, delay, cp, g, method="saft", interp="line
                               float(cp), _p(at), _p(ar), _p(g), _p(gY), _p(gat), _p(gar),
                               ctypes.byref(gcp), int(nthreads))
     return gY, gat, (gar[:Lr] if Lr else None), gcp.value
+
+
+# ----------------------------------------------------------------- convolution
+
+def fftconv_p(A, B, Y=None, mode="="):
+    """Column-wise linear convolution by zero-padded FFTs of length
+    M_A + M_B - 1, the algorithm of fftconv (extra_src/fftconv.cc:32-106)
+    driven by FFTConvP::run (extra_src/fftconv_p.cc:154-403): a vector B is
+    one filter for every column of A.  mode '=', '+=' or '-=' as ConvMode."""
+    A = np.asarray(A, dtype=np.float64)
+    A = A.reshape(A.shape[0], -1)
+    B = np.asarray(B, dtype=np.float64)
+    vec = B.ndim == 1 or B.shape[0] == 1 or B.shape[1] == 1
+    B = B.reshape(-1, 1) if vec else B
+    n = A.shape[0] + B.shape[0] - 1
+    Z = np.fft.irfft(np.fft.rfft(A, n, axis=0) * np.fft.rfft(B, n, axis=0), n, axis=0)
+    if Y is None or mode == "=":
+        return Z
+    return Y + Z if mode == "+=" else Y - Z
+
+
+def sum_fftconv(H, U):
+    """sum_l H(:,n,l) (*) U(:,l) by FFTs, accumulated in the frequency domain
+    as add_fftconv does (extra_src/fftconv.cc:119-209)."""
+    H = np.asarray(H, dtype=np.float64)
+    U = np.asarray(U, dtype=np.float64).reshape(U.shape[0], -1)
+    M, N, L = H.shape
+    n = M + U.shape[0] - 1
+    F = np.zeros((n // 2 + 1, N), dtype=np.complex128)
+    for l in range(L):
+        F += np.fft.rfft(H[:, :, l], n, axis=0) * np.fft.rfft(U[:, l], n)[:, None]
+    return np.fft.irfft(F, n, axis=0)
