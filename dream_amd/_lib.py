@@ -37,6 +37,8 @@ EXPORTS = (
     "dream_hip_das", "dream_hip_das_dev", "dream_hip_das_adjoint_dev",
     "dream_hip_das_batch_create", "dream_hip_das_batch_run", "dream_hip_das_batch_destroy",
     "dream_hip_fftconv_p", "dream_hip_sum_fftconv", "dream_hip_conv_sum_dev",
+    "dream_hip_das_uni_create", "dream_hip_das_uni_destroy", "dream_hip_das_uni_shape",
+    "dream_hip_das_uni_setup_has_changed", "dream_hip_das_uni", "dream_hip_das_uni_dev",
 )
 
 _i32, _i64, _dbl = ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -130,6 +132,18 @@ def _declare(L):
     L.dream_hip_das_dev.restype = _i32
     L.dream_hip_das_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _dbl, _i32, _vp, _dbl, _vp, _vp,
                                     _i32, _i64, _i64, _i64, _vp, _i32, _ip, _vp]
+    L.dream_hip_das_uni_create.restype = _i32
+    L.dream_hip_das_uni_create.argtypes = [_i32, _i64, _dp, _dp, _dp, _i32, ctypes.POINTER(_vp)]
+    L.dream_hip_das_uni_destroy.restype = _i32
+    L.dream_hip_das_uni_destroy.argtypes = [_vp]
+    L.dream_hip_das_uni_shape.restype = _i32
+    L.dream_hip_das_uni_shape.argtypes = [_vp, _i64p, _i64p, _i64p]
+    L.dream_hip_das_uni_setup_has_changed.restype = _i32
+    L.dream_hip_das_uni_setup_has_changed.argtypes = [_vp, _i32, _i64, _dp, _dp, _dp, _i32]
+    L.dream_hip_das_uni.restype = _i32
+    L.dream_hip_das_uni.argtypes = [_vp, _vp, _dbl, _dbl, _dbl, _vp]
+    L.dream_hip_das_uni_dev.restype = _i32
+    L.dream_hip_das_uni_dev.argtypes = [_vp, _vp, _dbl, _dbl, _dbl, _i32, _vp, _vp]
     L.dream_hip_fftconv_p.restype = _i32
     L.dream_hip_fftconv_p.argtypes = [_vp, _vp, _i64, _i64, _vp, _i64, _i64, _i32]
     L.dream_hip_sum_fftconv.restype = _i32

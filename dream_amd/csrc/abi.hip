@@ -362,6 +362,68 @@ int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const
   });
 }
 
+// ------------------------------------------------------------------ das_uni
+
+int dream_hip_das_uni_create(int das_type, int64_t a_scan_len, const double *gt,
+                             const double *gr, const double *Ro, int precision, void **handle)
+{
+  return guarded([&]() {
+    require(handle && gt && Ro && (das_type == 0 || gr), "das_uni_create: null argument");
+    require(a_scan_len >= 1 && a_scan_len < (1ll << 26),
+            "das_uni: A-scan length out of range (1 .. 2^26)");
+    *handle = new DasUni(das_type, a_scan_len, gt, das_type == 0 ? gt : gr, Ro, precision);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_uni_destroy(void *handle)
+{
+  return guarded([&]() {
+    delete static_cast<DasUni *>(handle);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_uni_shape(void *handle, int64_t *No, int64_t *num_t, int64_t *num_r)
+{
+  return guarded([&]() {
+    require(handle != nullptr, "das_uni: null handle");
+    const DasUni *u = static_cast<const DasUni *>(handle);
+    if (No) *No = u->No();
+    if (num_t) *num_t = u->Lt();
+    if (num_r) *num_r = u->Lr();
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_uni_setup_has_changed(void *handle, int das_type, int64_t a_scan_len,
+                                        const double *gt, const double *gr, const double *Ro,
+                                        int precision)
+{
+  if (!handle || !gt || !Ro || (das_type != 0 && !gr)) return 1;
+  return static_cast<DasUni *>(handle)->changed(das_type, a_scan_len, gt, das_type == 0 ? gt : gr,
+                                                Ro, precision) ? 1 : 0;
+}
+
+int dream_hip_das_uni(void *handle, const void *Y, double dt, double delay, double cp, void *Im)
+{
+  return guarded([&]() {
+    require(handle && Y && Im, "das_uni: null argument");
+    static_cast<DasUni *>(handle)->run_host(Y, dt, delay, cp, Im);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_uni_dev(void *handle, const void *Y, double dt, double delay, double cp,
+                          int interp, void *Im, void *stream)
+{
+  return guarded([&]() {
+    require(handle && Y && Im, "das_uni: null argument");
+    static_cast<DasUni *>(handle)->run_dev(Y, dt, delay, cp, interp, Im, (hipStream_t) stream);
+    return DREAM_HIP_OK;
+  });
+}
+
 // ------------------------------------------------------------------ convolution
 
 int dream_hip_conv_sum_dev(void *Y, const void *A, int64_t M_A, int64_t N, int64_t L,

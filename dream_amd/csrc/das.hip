@@ -258,9 +258,11 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
   } else {
     GT gy_c = GT(0), gx_r = GT(0);
     if (TYPE == DAS_RCA_COLTX || TYPE == DAS_RCA_ROWTX) {
-      gy_c = stripe_gap<GT>(yo, Gr[Lr], Gr[2 * Lr - 1]);
-      gx_r = stripe_gap<GT>(xo, Gt[0], Gt[Lt - 1]);
+      const bool given = A.stripe_given != 0;
+      gy_c = stripe_gap<GT>(yo, given ? (GT) A.sy_lo : Gr[Lr], given ? (GT) A.sy_hi : Gr[2 * Lr - 1]);
+      gx_r = stripe_gap<GT>(xo, given ? (GT) A.sx_lo : Gt[0], given ? (GT) A.sx_hi : Gt[Lt - 1]);
     }
+    const bool delay_rx = A.delay_rx != 0;
     const int nchunks = (Lr + kRC - 1) / kRC;
     const int J = nchunks * ((Lt + kTB - 1) / kTB);
     for (int job = job0; job < J; job += jstride) {
@@ -275,6 +277,7 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
           tr[jj] = leg3<GT>(Gr[j] - xo, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
         else
           tr[jj] = leg3<GT>(gx_r, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
+        if (delay_rx) tr[jj] += dl;
         tr[jj] = clamp_leg<GT>(tr[jj], tmax);
       }
       // Extremes of the chunk's receive legs: a transmit row whose sample
@@ -292,7 +295,7 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
           tt = leg3<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
         else
           tt = leg3<GT>(Gt[i] - xo, gy_c, Gt[i + 2 * Lt] - zo) * inv_cp;
-        tt += dl;
+        if (!delay_rx) tt += dl;
         tt = clamp_leg<GT>(tt, tmax);
         const bool row_out = !valid || !((tt + trmin) * fs < (GT) K) || !((tt + trmax) * fs > GT(-1));
         if (__all(row_out)) {
@@ -1161,6 +1164,25 @@ void batch_dispatch_type(int type, int interp, bool apod, const DasArgs *args,
   }
 }
 
+// Pixels of a uniform grid in das_uni's linear order (y fastest, then z,
+// then x; opencl/das_uni_double.cl:32-47): x = min_x + ((T) nx) * dx etc.
+template <typename GT>
+__global__ void uni_grid_kernel(GT *Ro, int64_t No, int64_t Nx, int64_t Ny, int64_t Nz, GT min_x,
+                                GT dx, GT min_y, GT dy, GT min_z, GT dz)
+{
+  const int64_t no = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (no >= No) return;
+  const int64_t yq = no / Ny;
+  const int64_t ny = no - yq * Ny;
+  const int64_t zq = yq / Nz;
+  const int64_t nz = yq - zq * Nz;
+  const int64_t xq = zq / Nx;
+  const int64_t nx = zq - xq * Nx;
+  Ro[no] = min_x + ((GT) nx) * dx;
+  Ro[no + No] = min_y + ((GT) ny) * dy;
+  Ro[no + 2 * No] = min_z + ((GT) nz) * dz;
+}
+
 size_t elem_size(int prec) { return prec == DREAM_HIP_F64 ? 8 : 4; }
 size_t geom_size(int prec) { return prec == DREAM_HIP_F32_REF ? 4 : 8; }
 
@@ -1277,6 +1299,11 @@ DasHandle::~DasHandle()
   (void) hipFree(d_z_);
   (void) hipFree(d_scratch_);
   (void) hipFree(d_perm_);
+}
+
+void DasHandle::ensure_scratch_for(int64_t n_frames)
+{
+  ensure_scratch(das_forward_scratch_bytes(type_, prec_, No_, n_frames, Lt_, Lr_));
 }
 
 void DasHandle::ensure_scratch(size_t bytes)
@@ -1502,6 +1529,172 @@ void DasHandle::adjoint_dev(const void *Y, const void *Ro, const void *Gt, const
                          (float *) g_cp);
   }
   DREAM_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ das_uni
+
+namespace {
+
+// Element / grid count (max - min)/pitch + 1 truncated, in T as the
+// reference computes it (include/das_uni.h:70-78, extra_src/oct_das_uni.cc:280-290).
+template <typename T>
+int64_t uni_count(double mn, double pitch, double mx)
+{
+  return (int64_t) ((double) (((T) mx - (T) mn) / (T) pitch) + 1.0);
+}
+
+template <typename GT>
+void uni_tables(int type, const double *gt, const double *gr, int64_t Lt, int64_t Lr,
+                std::vector<char> &out)
+{
+  // Transmit (SAFT/TFM) or column (RCA) elements along x; receive (TFM)
+  // elements along x, or row (RCA) elements along y; all at z = 0.
+  const int64_t Lrr = type == DAS_SAFT ? 0 : Lr;
+  std::vector<GT> g((size_t) (3 * (Lt + Lrr)), GT(0));
+  for (int64_t n = 0; n < Lt; n++) g[(size_t) n] = (GT) gt[0] + ((GT) n) * (GT) gt[1];
+  GT *r = g.data() + 3 * Lt;
+  for (int64_t n = 0; n < Lrr; n++) {
+    const GT v = (GT) gr[0] + ((GT) n) * (GT) gr[1];
+    if (type == DAS_TFM) r[n] = v; else r[n + Lrr] = v;
+  }
+  out.assign((const char *) g.data(), (const char *) g.data() + g.size() * sizeof(GT));
+}
+
+} // namespace
+
+DasUni::DasUni(int type, int64_t K, const double gt[3], const double gr[3], const double Ro[9],
+               int precision)
+  : type_(type), prec_(precision), K_(K)
+{
+  require(type >= DAS_SAFT && type <= DAS_RCA_ROWTX, "das_uni: unknown DAS type");
+  require(precision == DREAM_HIP_F64 || precision == DREAM_HIP_F32_REF ||
+              precision == DREAM_HIP_F32,
+          "das_uni: unknown precision");
+  std::memcpy(gt_, gt, sizeof gt_);
+  std::memcpy(gr_, type == DAS_SAFT ? gt : gr, sizeof gr_);
+  std::memcpy(ro_, Ro, sizeof ro_);
+  const bool f = precision == DREAM_HIP_F32_REF;
+  gs_ = f ? 4 : 8;
+  auto count = [&](double mn, double p, double mx) {
+    require(p > 0 && mx >= mn, "das_uni: need pitch > 0 and max >= min");
+    return f ? uni_count<float>(mn, p, mx) : uni_count<double>(mn, p, mx);
+  };
+  Lt_ = count(gt[0], gt[1], gt[2]);
+  Lr_ = type == DAS_SAFT ? 0 : count(gr[0], gr[1], gr[2]);
+  const int64_t Nx = count(Ro[0], Ro[3], Ro[6]), Ny = count(Ro[1], Ro[4], Ro[7]),
+                Nz = count(Ro[2], Ro[5], Ro[8]);
+  require(Nx >= 1 && Ny >= 1 && Nz >= 1 && Lt_ >= 1, "das_uni: empty grid");
+  No_ = Nx * Ny * Nz;
+  h_ = new DasHandle(type, K, No_, Lt_, Lr_, precision);
+  // Device geometry: Ro, then the element tables, then the delay.
+  std::vector<char> tabs;
+  if (f) uni_tables<float>(type, gt, gr, Lt_, Lr_, tabs);
+  else uni_tables<double>(type, gt, gr, Lt_, Lr_, tabs);
+  const size_t ro_bytes = (size_t) No_ * 3 * gs_;
+  DREAM_HIP_CHECK(hipMalloc(&d_geom_, ro_bytes + tabs.size() + 16));
+  char *base = (char *) d_geom_;
+  DREAM_HIP_CHECK(hipMemcpy(base + ro_bytes, tabs.data(), tabs.size(), hipMemcpyHostToDevice));
+  const int blocks = (int) ((No_ + 255) / 256);
+  if (f)
+    hipLaunchKernelGGL(uni_grid_kernel<float>, dim3(blocks), dim3(256), 0, lib_stream(),
+                       (float *) base, No_, Nx, Ny, Nz, (float) Ro[0], (float) Ro[3],
+                       (float) Ro[1], (float) Ro[4], (float) Ro[2], (float) Ro[5]);
+  else
+    hipLaunchKernelGGL(uni_grid_kernel<double>, dim3(blocks), dim3(256), 0, lib_stream(),
+                       (double *) base, No_, Nx, Ny, Nz, Ro[0], Ro[3], Ro[1], Ro[4], Ro[2], Ro[5]);
+  DREAM_HIP_CHECK(hipGetLastError());
+  DREAM_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  // Visit the pixels in Morton order of their grid indices.
+  std::vector<double> idx((size_t) No_ * 3);
+  for (int64_t no = 0; no < No_; no++) {
+    const int64_t yq = no / Ny, zq = yq / Nz;
+    idx[(size_t) no] = (double) (zq % Nx);
+    idx[(size_t) (no + No_)] = (double) (no % Ny);
+    idx[(size_t) (no + 2 * No_)] = (double) (yq % Nz);
+  }
+  std::vector<int32_t> perm((size_t) No_);
+  pixel_order<double>(idx.data(), No_, perm.data());
+  h_->set_pixel_order(perm.data(), No_);
+}
+
+DasUni::~DasUni()
+{
+  delete h_;
+  (void) hipFree(d_geom_);
+  (void) hipFree(d_buf_);
+}
+
+bool DasUni::changed(int type, int64_t K, const double gt[3], const double gr[3],
+                     const double Ro[9], int precision) const
+{
+  if (type != type_ || K != K_ || precision != prec_) return true;
+  for (int i = 0; i < 3; i++)
+    if (gt[i] != gt_[i] || (type != DAS_SAFT && gr[i] != gr_[i])) return true;
+  for (int i = 0; i < 9; i++)
+    if (Ro[i] != ro_[i]) return true;
+  return false;
+}
+
+void DasUni::run_dev(const void *Y, double dt, double delay, double cp, int interp, void *Im,
+                     hipStream_t s)
+{
+  if (cancelled()) throw Error(DREAM_HIP_ECANCELLED, "das_uni cancelled");
+  char *base = (char *) d_geom_;
+  const size_t ro_bytes = (size_t) No_ * 3 * gs_;
+  const void *Gt = base + ro_bytes;
+  const void *Gr = base + ro_bytes + (size_t) Lt_ * 3 * gs_;
+  void *dd = base + ro_bytes + (size_t) (Lt_ + (type_ == DAS_SAFT ? 0 : Lr_)) * 3 * gs_;
+  // The scalar delay in the geometry type, stream-ordered before the launch.
+  if (gs_ == 8) {
+    DREAM_HIP_CHECK(hipMemcpyAsync(dd, &delay, 8, hipMemcpyHostToDevice, s));
+  } else {
+    const float df = (float) delay;
+    DREAM_HIP_CHECK(hipMemcpyAsync(dd, &df, 4, hipMemcpyHostToDevice, s));
+  }
+  DasArgs a;
+  h_->fill_args(a, Y, base, Gt, type_ == DAS_SAFT ? nullptr : Gr, dt, 0, dd, cp, nullptr,
+                nullptr, 1, 0, 0, Im);
+  if (type_ == DAS_RCA_COLTX || type_ == DAS_RCA_ROWTX) {
+    // The stripe extents are the [min, max] parameters (opencl/das_uni_double.cl:197-200).
+    a.stripe_given = 1;
+    a.sx_lo = gs_ == 8 ? gt_[0] : (double) (float) gt_[0];
+    a.sx_hi = gs_ == 8 ? gt_[2] : (double) (float) gt_[2];
+    a.sy_lo = gs_ == 8 ? gr_[0] : (double) (float) gr_[0];
+    a.sy_hi = gs_ == 8 ? gr_[2] : (double) (float) gr_[2];
+    // Row transmit: the delay is on the row (receive-index) leg
+    // (opencl/das_uni_double.cl:344-346).
+    a.delay_rx = type_ == DAS_RCA_ROWTX;
+  }
+  a.y_stride = (type_ == DAS_SAFT ? Lt_ : Lt_ * Lr_) * K_;
+  a.im_stride = No_;
+  a.status = nullptr;
+  h_->ensure_scratch_for(1);
+  a.scratch = h_->scratch();
+  a.scratch_bytes = h_->scratch_bytes();
+  require(interp == DREAM_HIP_NEAREST || interp == DREAM_HIP_LINEAR, "das_uni: unknown interpolation");
+  launch_das_forward(a, type_, prec_, interp, false, s);
+}
+
+void DasUni::run_host(const void *Y, double dt, double delay, double cp, void *Im)
+{
+  const size_t es = prec_ == DREAM_HIP_F64 ? 8 : 4;
+  const size_t bY = (size_t) ((type_ == DAS_SAFT ? Lt_ : Lt_ * Lr_) * K_) * es;
+  const size_t bIm = (size_t) No_ * es;
+  const size_t total = ((bY + 255) & ~(size_t) 255) + bIm;
+  if (total > d_buf_bytes_) {
+    (void) hipFree(d_buf_);
+    d_buf_ = nullptr;
+    d_buf_bytes_ = 0;
+    if (hipMalloc(&d_buf_, total) != hipSuccess)
+      throw Error(DREAM_HIP_ENOMEM, "das_uni: device allocation failed");
+    d_buf_bytes_ = total;
+  }
+  char *dY = (char *) d_buf_, *dIm = dY + ((bY + 255) & ~(size_t) 255);
+  hipStream_t s = lib_stream();
+  DREAM_HIP_CHECK(hipMemcpyAsync(dY, Y, bY, hipMemcpyHostToDevice, s));
+  run_dev(dY, dt, delay, cp, DREAM_HIP_NEAREST, dIm, s);
+  DREAM_HIP_CHECK(hipMemcpyAsync(Im, dIm, bIm, hipMemcpyDeviceToHost, s));
+  DREAM_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // ------------------------------------------------------------------ batch

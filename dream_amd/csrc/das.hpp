@@ -18,6 +18,12 @@ struct DasArgs {
   double fs_khz, inv_cp, delay_ms, cp;   // in the geometry type, passed as double
   double leg_max;                        // legs are clamped to +-leg_max (2^29 / fs)
   const void *apod_t, *apod_r;
+  // RCA only: stripe extents given explicitly (sx: column x range for the
+  // row leg, sy: row y range for the column leg) instead of read from the
+  // first and last elements; and the delay on the receive leg (das_uni's
+  // row-transmit rule) instead of the transmit leg.
+  int32_t stripe_given, delay_rx;
+  double sx_lo, sx_hi, sy_lo, sy_hi;
   int64_t n_frames, y_stride, im_stride;
   void *Im;
   Status *status;
@@ -63,12 +69,16 @@ public:
 
   int type() const { return type_; }
   int precision() const { return prec_; }
-
-private:
+  // Forward scratch for launches of n_frames frames, and its current extent.
+  void ensure_scratch_for(int64_t n_frames);
+  void *scratch() const { return d_scratch_; }
+  size_t scratch_bytes() const { return d_scratch_bytes_; }
   void fill_args(DasArgs &a, const void *Y, const void *Ro, const void *Gt, const void *Gr,
                  double dt, int delay_type, const void *delay, double cp, const void *apod_t,
                  const void *apod_r, int64_t n_frames, int64_t y_stride, int64_t im_stride,
                  void *Im) const;
+
+private:
   int type_;           // effective type (SAFT when Lr == 0)
   int64_t K_, No_, Lt_, Lr_;
   int prec_;
@@ -93,6 +103,36 @@ private:
 // fs_khz, inv_cp, delay_ms (0), cp and leg_max of a launch, with the
 // reference's rounding of the constants to T (extra_src/das.cc:316-318).
 void das_constants(DasArgs &a, int prec, double dt, double cp);
+
+// das_uni: DAS over implicit uniform grids.  The array elements and the
+// pixels are generated on the device from [min, pitch, max] triples with the
+// reference's formulas (opencl/das_uni_double.cl), then the DAS kernels run
+// on them.
+class DasUni {
+public:
+  DasUni(int type, int64_t K, const double gt[3], const double gr[3], const double Ro[9],
+         int precision);
+  ~DasUni();
+  bool changed(int type, int64_t K, const double gt[3], const double gr[3], const double Ro[9],
+               int precision) const;
+  // Device pointers; Im (No) is overwritten.
+  void run_dev(const void *Y, double dt, double delay, double cp, int interp, void *Im,
+               hipStream_t s);
+  // Host pointers (copy in, beamform, copy out).
+  void run_host(const void *Y, double dt, double delay, double cp, void *Im);
+  int64_t No() const { return No_; }
+  int64_t Lt() const { return Lt_; }
+  int64_t Lr() const { return Lr_; }
+private:
+  int type_, prec_;
+  int64_t K_, No_ = 0, Lt_ = 0, Lr_ = 0;
+  double gt_[3], gr_[3], ro_[9];
+  DasHandle *h_ = nullptr;
+  void *d_geom_ = nullptr;            // Ro (No x 3), Gt (Lt x 3), Gr (Lr x 3), delay
+  void *d_buf_ = nullptr;             // host-path copies of Y and Im
+  size_t d_buf_bytes_ = 0;
+  size_t gs_ = 8;
+};
 
 // Many problems of one DAS type, precision and sampling rule in one
 // persistent forward launch (device pointers; scalar delays).

@@ -246,6 +246,37 @@ int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const
                               int64_t im_frame_stride, const void *g, void *gY,
                               void *g_apod_t, void *g_apod_r, void *g_cp, void *stream);
 
+/* ---------------------------------------------------------------- das_uni */
+
+/* DAS over implicit uniform grids: DAS_UNI<T>::DAS_UNI / cl_das_uni
+ * (include/das_uni.h:50-365, opencl/das_uni_double.cl).  gt = [min_t,
+ * pitch_t, max_t] (transmit elements along x, or the RCA columns), gr = [min_r,
+ * pitch_r, max_r] (receive elements along x, or the RCA rows along y;
+ * ignored for SAFT), Ro = the 3 x 3 column-major matrix [min_Rx dx max_Rx;
+ * min_Ry dy max_Ry; min_Rz dz max_Rz] [mm].  Counts are (max - min)/pitch + 1
+ * truncated; pixel n is (ny fastest, then nz, then nx).  The geometry is
+ * generated on the device once per handle.  The reference kernel computes
+ * in OpenCL with native_sqrt; here the arithmetic is das's (correctly rounded
+ * sqrt, no contraction), with das_uni's two RCA rules: the stripe extents are
+ * the [min, max] parameters, and for row transmit the delay is on the row
+ * leg.  precision as dream_hip_das_create. */
+int dream_hip_das_uni_create(int das_type, int64_t a_scan_len, const double *gt,
+                             const double *gr, const double *Ro, int precision, void **handle);
+int dream_hip_das_uni_destroy(void *handle);
+/* Pixel count and element counts of a handle (any pointer may be NULL). */
+int dream_hip_das_uni_shape(void *handle, int64_t *No, int64_t *num_t, int64_t *num_r);
+/* DAS_UNI<T>::das_setup_has_changed: 1 if the arguments differ from the handle's. */
+int dream_hip_das_uni_setup_has_changed(void *handle, int das_type, int64_t a_scan_len,
+                                        const double *gt, const double *gr, const double *Ro,
+                                        int precision);
+/* cl_das_uni: HOST Y (K x n_ascans of the handle's element type) and Im (No);
+ * scalar delay [us]; returns after Im is written. */
+int dream_hip_das_uni(void *handle, const void *Y, double dt, double delay, double cp, void *Im);
+/* Device form (DEVICE Y and Im, asynchronous on `stream`), with the
+ * sampling rule (DREAM_HIP_NEAREST = the reference's, or DREAM_HIP_LINEAR). */
+int dream_hip_das_uni_dev(void *handle, const void *Y, double dt, double delay, double cp,
+                          int interp, void *Im, void *stream);
+
 /* ---------------------------------------------------------------- convolution */
 
 /* Convolution modes (ConvMode, include/dream.h:91-95). */

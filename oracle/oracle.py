@@ -313,3 +313,88 @@ def sum_fftconv(H, U):
     for l in range(L):
         F += np.fft.rfft(H[:, :, l], n, axis=0) * np.fft.rfft(U[:, l], n)[:, None]
     return np.fft.irfft(F, n, axis=0)
+
+
+# ----------------------------------------------------------------- das_uni
+
+def uni_count(mn, pitch, mx, dtype=np.float64):
+    """(max - min)/pitch + 1 truncated, with the subtraction and division in T
+    (include/das_uni.h:70-78)."""
+    T = dtype
+    return int(float((T(mx) - T(mn)) / T(pitch)) + 1.0)
+
+
+def uni_pixels(Ro3, dtype=np.float64):
+    """Pixels of das_uni's grid in its linear order: ny fastest, then nz, then
+    nx; x = min + ((T) n) * d (opencl/das_uni_double.cl:32-47)."""
+    T = dtype
+    R = np.asarray(Ro3, dtype=np.float64).reshape(3, 3, order="F")
+    N = [uni_count(R[c, 0], R[c, 1], R[c, 2], T) for c in range(3)]
+    no = np.arange(N[0] * N[1] * N[2])
+    ny = no % N[1]
+    nz = (no // N[1]) % N[2]
+    nx = (no // N[1] // N[2]) % N[0]
+    x = T(R[0, 0]) + nx.astype(T) * T(R[0, 1])
+    y = T(R[1, 0]) + ny.astype(T) * T(R[1, 1])
+    z = T(R[2, 0]) + nz.astype(T) * T(R[2, 1])
+    return np.asarray(x, T), np.asarray(y, T), np.asarray(z, T)
+
+
+def das_uni(Y, gt, gr, Ro3, dt, delay, cp, method="tfm", precision="f64"):
+    """Restatement of the das_uni kernels (opencl/das_uni_double.cl:9-394) with
+    IEEE sqrt in place of native_sqrt, vectorised over the pixels.  Y is
+    K x n_ascans (Fortran order); returns Im (No,)."""
+    T = np.float64 if precision == "f64" else np.float32
+    Y = np.asarray(Y, dtype=T)
+    K = Y.shape[0]
+    xo, yo, zo = uni_pixels(Ro3, T)
+    # The scalars arrive as T (extra_src/oct_das_uni.cc:300-340); the
+    # constants are double expressions of them rounded to T.
+    dt, delay, cp = float(T(dt)), float(T(delay)), float(T(cp))
+    Fs = T((1.0 / dt) * 1000.0)
+    icp = T(1.0 / cp)
+    dms = T(delay / 1000.0)
+    gt = [float(v) for v in gt]
+    gr = gt if method == "saft" else [float(v) for v in gr]
+    Lt = uni_count(*gt, dtype=T)
+    Lr = uni_count(*gr, dtype=T)
+    et = T(gt[0]) + np.arange(Lt).astype(T) * T(gt[1])
+    er = T(gr[0]) + np.arange(Lr).astype(T) * T(gr[1])
+    im = np.zeros(xo.shape, dtype=T)
+
+    def gather(a, t_dp):
+        k = (t_dp * Fs).astype(np.int64)                  # (int) truncation
+        ok = (k >= 0) & (k < K)
+        return np.where(ok, Y[np.clip(k, 0, K - 1), a], T(0))
+
+    def leg(gx, gy, gz):
+        return np.sqrt((gx * gx + gy * gy) + gz * gz) * icp
+
+    def stripe(p, lo, hi):
+        lo, hi = T(lo), T(hi)
+        return np.where((p >= lo) & (p <= hi), T(0), np.where(p < lo, lo - p, hi - p)).astype(T)
+
+    if method == "saft":
+        for n in range(Lt):
+            t = leg(et[n] - xo, yo, zo) + dms
+            im = im + gather(n, T(2.0) * t)
+    elif method == "tfm":
+        for i in range(Lt):
+            tt = leg(et[i] - xo, yo, zo) + dms
+            for j in range(Lr):
+                im = im + gather(i * Lr + j, tt + leg(er[j] - xo, yo, zo))
+    elif method in ("rca", "rca_coltx"):
+        gy_c = stripe(yo, gr[0], gr[2])                   # rows' [min, max] (:197-200)
+        gx_r = stripe(xo, gt[0], gt[2])
+        for c in range(Lt):
+            tc = leg(et[c] - xo, gy_c, zo) + dms
+            for r in range(Lr):
+                im = im + gather(c * Lr + r, tc + leg(gx_r, er[r] - yo, zo))
+    else:                                                 # rca_rowtx
+        gy_c = stripe(yo, gr[0], gr[2])
+        gx_r = stripe(xo, gt[0], gt[2])
+        for r in range(Lr):
+            tr = leg(gx_r, er[r] - yo, zo) + dms          # delay on the row leg (:344-346)
+            for c in range(Lt):
+                im = im + gather(r * Lt + c, leg(et[c] - xo, gy_c, zo) + tr)
+    return im
