@@ -39,6 +39,7 @@ EXPORTS = (
     "dream_hip_fftconv_p", "dream_hip_sum_fftconv", "dream_hip_conv_sum_dev",
     "dream_hip_das_uni_create", "dream_hip_das_uni_destroy", "dream_hip_das_uni_shape",
     "dream_hip_das_uni_setup_has_changed", "dream_hip_das_uni", "dream_hip_das_uni_dev",
+    "dream_hip_arr_circ", "dream_hip_circ", "dream_hip_arr_circ_dev",
 )
 
 _i32, _i64, _dbl = ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -63,6 +64,7 @@ class DreamSirProblem(ctypes.Structure):
         ("apod_par", _dbl),
         ("err_level", ctypes.c_int32),
         ("h", _vp),
+        ("shape", ctypes.c_int32), ("R", _dbl),
     ]
 
 
@@ -104,6 +106,16 @@ def _declare(L):
     L.dream_hip_rect.argtypes = [_dbl] + sir_common + [_vp, _i32, _ip]
     L.dream_hip_arr_rect_dev.restype = _i32
     L.dream_hip_arr_rect_dev.argtypes = sir_common + [
+        _i64, _vp, _i32, _vp, _i32, _dbl, _dbl, _vp, _i32, _i32, _dbl, _vp, _i32, _i32, _ip,
+        _i64p, _vp]
+    L.dream_hip_arr_circ.restype = _i32
+    L.dream_hip_arr_circ.argtypes = [_dbl, _vp, _i64] + [_dbl] * 4 + [_i64, _i32, _vp, _dbl, _dbl] + [
+        _i64, _vp, _i32, _vp, _i32, _dbl, _dbl, _vp, _i32, _i32, _dbl, _vp, _i32, _ip]
+    L.dream_hip_circ.restype = _i32
+    L.dream_hip_circ.argtypes = [_dbl, _vp, _i64] + [_dbl] * 4 + [_i64, _i32, _vp, _dbl, _dbl,
+                                                                _vp, _i32, _ip]
+    L.dream_hip_arr_circ_dev.restype = _i32
+    L.dream_hip_arr_circ_dev.argtypes = [_vp, _i64] + [_dbl] * 4 + [_i64, _i32, _vp, _dbl, _dbl] + [
         _i64, _vp, _i32, _vp, _i32, _dbl, _dbl, _vp, _i32, _i32, _dbl, _vp, _i32, _i32, _ip,
         _i64p, _vp]
     L.dream_hip_rect_indices.restype = _i32

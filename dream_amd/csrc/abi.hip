@@ -80,6 +80,20 @@ SirProblem make_problem(const double *Ro, int64_t No, double a, double b, double
                         double dt, int64_t nt, int delay_type, const double *delay, double v,
                         double cp, int64_t L, const double *G, int foc_met, const double *focal,
                         int steer_met, double theta, double phi, const double *apod,
+                        int do_apod, int apod_met, double apod_par, void *h, int err_level);
+
+// A disc problem: radius R in place of the rectangle's sides.
+SirProblem make_circ_problem(const double *Ro, int64_t No, double R, double dx, double dy,
+                             double dt, int64_t nt, int delay_type, const double *delay, double v,
+                             double cp, int64_t L, const double *G, int foc_met,
+                             const double *focal, int steer_met, double theta, double phi,
+                             const double *apod, int do_apod, int apod_met, double apod_par,
+                             void *h, int err_level);
+
+SirProblem make_problem(const double *Ro, int64_t No, double a, double b, double dx, double dy,
+                        double dt, int64_t nt, int delay_type, const double *delay, double v,
+                        double cp, int64_t L, const double *G, int foc_met, const double *focal,
+                        int steer_met, double theta, double phi, const double *apod,
                         int do_apod, int apod_met, double apod_par, void *h, int err_level)
 {
   require(L >= 1 && G, "dream_arr_rect: need a grid G with at least one element");
@@ -116,6 +130,57 @@ SirProblem make_problem(const double *Ro, int64_t No, double a, double b, double
   p.err_level = err_level;
   p.h = h;
   return p;
+}
+
+SirProblem make_circ_problem(const double *Ro, int64_t No, double R, double dx, double dy,
+                             double dt, int64_t nt, int delay_type, const double *delay, double v,
+                             double cp, int64_t L, const double *G, int foc_met,
+                             const double *focal, int steer_met, double theta, double phi,
+                             const double *apod, int do_apod, int apod_met, double apod_par,
+                             void *h, int err_level)
+{
+  SirProblem p = make_problem(Ro, No, 0.0, 0.0, dx, dy, dt, nt, delay_type, delay, v, cp, L, G,
+                              foc_met, focal, steer_met, theta, phi, apod, do_apod, apod_met,
+                              apod_par, h, err_level);
+  p.shape = SIR_CIRC;
+  p.R = R;
+  return p;
+}
+
+// One SIR problem on the device: run a single-problem plan, then read its status.
+void run_single(const SirProblem &p, int out_dtype, hipStream_t s, int *sir_err,
+                int64_t *oob_count)
+{
+  std::vector<SirProblem> probs{p};
+  SirPlan plan(probs, out_dtype);
+  plan.run(s, false);
+  int32_t code = 0;
+  int64_t oob = 0;
+  plan.status(&code, &oob);   // synchronises `s`
+  if (sir_err) *sir_err = code;
+  if (oob_count) *oob_count = oob;
+}
+
+// Host-pointer SIR of one problem (FP64 output): copy in, run, copy out.
+int run_host_sir(SirProblem p, double alpha, const double *Ro, const double *delay, double *h,
+                 int *sir_err)
+{
+  if (alpha > 2.220446049250313e-16)
+    throw Error(DREAM_HIP_EUNSUPPORTED, "SIR: attenuation (alpha > 0) is not on the GPU path");
+  require(p.No >= 0 && p.nt >= 1, "SIR: need No >= 0 and nt >= 1");
+  require(p.No == 0 || (Ro && delay && h), "SIR: null argument");
+  hipStream_t s = lib_stream();
+  const int64_t nd = p.delay_type ? p.No : 1;
+  DevBuf dRo(Ro, (size_t) p.No * 3 * 8, s), dD(delay, (size_t) nd * 8, s);
+  DevBuf dH((size_t) (p.No * p.nt) * 8);
+  p.Ro = (const double *) dRo.p;
+  p.delay = (const double *) dD.p;
+  p.h = dH.p;
+  int code = 0;
+  run_single(p, DREAM_HIP_F64, s, &code, nullptr);
+  if (p.No) DREAM_HIP_CHECK(hipMemcpy(h, dH.p, (size_t) (p.No * p.nt) * 8, hipMemcpyDeviceToHost));
+  if (sir_err) *sir_err = code;
+  return DREAM_HIP_OK;
 }
 
 } // namespace dream
@@ -206,6 +271,49 @@ int dream_hip_rect(double alpha, const double *Ro, int64_t No, double a, double 
                             0, &focal, 0, 0.0, 0.0, nullptr, 0, 0, 0.0, h, err_level, sir_err);
 }
 
+int dream_hip_arr_circ_dev(const double *Ro, int64_t No, double R, double dx, double dy,
+                           double dt, int64_t nt, int delay_type, const double *delay, double v,
+                           double cp, int64_t num_elements, const double *G, int foc_met,
+                           const double *focal, int steer_met, double theta, double phi,
+                           const double *apod, int do_apod, int apod_met, double apod_par,
+                           void *h, int out_dtype, int err_level, int *sir_err,
+                           int64_t *oob_count, void *stream)
+{
+  return guarded([&]() {
+    run_single(make_circ_problem(Ro, No, R, dx, dy, dt, nt, delay_type, delay, v, cp,
+                                 num_elements, G, foc_met, focal, steer_met, theta, phi, apod,
+                                 do_apod, apod_met, apod_par, h, err_level),
+               out_dtype, (hipStream_t) stream, sir_err, oob_count);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_arr_circ(double alpha, const double *Ro, int64_t No, double R, double dx,
+                       double dy, double dt, int64_t nt, int delay_type, const double *delay,
+                       double v, double cp, int64_t num_elements, const double *G, int foc_met,
+                       const double *focal, int steer_met, double theta, double phi,
+                       const double *apod, int do_apod, int apod_met, double apod_par,
+                       double *h, int err_level, int *sir_err)
+{
+  return guarded([&]() {
+    return run_host_sir(make_circ_problem(nullptr, No, R, dx, dy, dt, nt, delay_type, nullptr, v,
+                                          cp, num_elements, G, foc_met, focal, steer_met, theta,
+                                          phi, apod, do_apod, apod_met, apod_par, nullptr,
+                                          err_level),
+                        alpha, Ro, delay, h, sir_err);
+  });
+}
+
+int dream_hip_circ(double alpha, const double *Ro, int64_t No, double R, double dx, double dy,
+                   double dt, int64_t nt, int delay_type, const double *delay, double v,
+                   double cp, double *h, int err_level, int *sir_err)
+{
+  const double G[3] = {0.0, 0.0, 0.0};
+  const double focal = 0.0;
+  return dream_hip_arr_circ(alpha, Ro, No, R, dx, dy, dt, nt, delay_type, delay, v, cp, 1, G, 0,
+                            &focal, 0, 0.0, 0.0, nullptr, 0, 0, 0.0, h, err_level, sir_err);
+}
+
 int dream_hip_rect_indices(double xo, double yo, double zo, double a, double b, double dx,
                            double dy, double dt, double delay, double cp, int64_t *it_out,
                            int64_t cap, int64_t *count)
@@ -227,10 +335,13 @@ int dream_hip_sir_plan_create(const DreamSirProblem *problems, int32_t n, int ou
     probs.reserve((size_t) n);
     for (int32_t i = 0; i < n; i++) {
       const DreamSirProblem &q = problems[i];
+      require(q.shape == SIR_RECT || q.shape == SIR_CIRC, "sir_plan_create: unknown shape");
       probs.push_back(make_problem(q.Ro, q.No, q.a, q.b, q.dx, q.dy, q.dt, q.nt, q.delay_type,
                                    q.delay, q.v, q.cp, q.num_elements, q.G, q.foc_met, q.focal,
                                    q.steer_met, q.theta, q.phi, q.apod, q.do_apod, q.apod_met,
                                    q.apod_par, q.h, q.err_level));
+      probs.back().shape = q.shape;
+      probs.back().R = q.R;
     }
     *plan = new SirPlan(probs, out_dtype);
     return DREAM_HIP_OK;

@@ -180,17 +180,27 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
         oob_last |= last;
       }
     };
-    // Sub-element x positions four at a time, so their scalar loads share
-    // one wait.
-    int i = 0;
-    for (; i + 4 <= nx; i += 4) {
-      const double x0 = xs[i], x1 = xs[i + 1], x2 = xs[i + 2], x3 = xs[i + 3];
-      contribution(x0);
-      contribution(x1);
-      contribution(x2);
-      contribution(x3);
+    if (P.shape == SIR_CIRC) {
+      // A disc row: the reference's accumulated walk from this row's start.
+      double x = ((gdouble *) P.row_x0)[row];
+      const int cnt = (int) ((gdouble *) P.row_n)[row];
+      for (int i = 0; i < cnt; i++) {
+        contribution(x);
+        x += P.dx;
+      }
+    } else {
+      // Sub-element x positions four at a time, so their scalar loads share
+      // one wait.
+      int i = 0;
+      for (; i + 4 <= nx; i += 4) {
+        const double x0 = xs[i], x1 = xs[i + 1], x2 = xs[i + 2], x3 = xs[i + 3];
+        contribution(x0);
+        contribution(x1);
+        contribution(x2);
+        contribution(x3);
+      }
+      for (; i < nx; i++) contribution(xs[i]);
     }
-    for (; i < nx; i++) contribution(xs[i]);
     e += P.step_e;
     j += P.step_j;
     if (j >= ny) { j -= ny; e++; }
@@ -319,6 +329,27 @@ void launch_fill_zero(void *p, size_t bytes, hipStream_t s)
 } // namespace
 
 // ------------------------------------------------------------------ host helpers
+
+void disc_rows(double R, double dx, double dy, std::vector<double> &ys, std::vector<double> &x0,
+               std::vector<double> &count)
+{
+  ys.clear();
+  x0.clear();
+  count.clear();
+  const double ysmin = -R, ysmax = R;
+  double y = ysmin + dy / 2.0;
+  while (y <= ysmax) {
+    const double rxs = std::sqrt(R * R - y * y);
+    const double xsmin = -rxs, xsmax = rxs;
+    const double x = xsmin + dx / 2.0;
+    double n = 0.0;
+    for (double xs = x; xs <= xsmax; xs += dx) n += 1.0;
+    ys.push_back(y);
+    x0.push_back(x);
+    count.push_back(n);
+    y += dy;
+  }
+}
 
 std::vector<double> subgrid(double w, double d)
 {
@@ -462,7 +493,7 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
   for (auto &p : problems) total_no += p.No;
 
   // Host tables and per-problem shape.
-  std::vector<std::vector<double>> xs(P), ys(P), el(P);
+  std::vector<std::vector<double>> xs(P), ys(P), el(P), rx0(P), rn(P);
   std::vector<SirDev> dev(P);
   std::vector<double> cost(P, 0.0);
   size_t table_doubles = 0;
@@ -474,10 +505,22 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
     require(L >= 1, "SIR: need at least one element");
     require(p.dx > 0 && p.dy > 0 && p.dt > 0, "SIR: dx, dy and dt must be positive");
     require(p.No == 0 || (p.Ro && p.delay && p.h), "SIR: null Ro, delay or h");
-    xs[i] = subgrid(p.a, p.dx);
-    ys[i] = subgrid(p.b, p.dy);
+    require(p.shape == SIR_RECT || p.shape == SIR_CIRC, "SIR: unknown element shape");
+    int64_t nx_cost;
+    if (p.shape == SIR_CIRC) {
+      disc_rows(p.R, p.dx, p.dy, ys[i], rx0[i], rn[i]);
+      double sum = 0.0, mx = 0.0;
+      for (double c : rn[i]) { sum += c; mx = std::max(mx, c); }
+      xs[i].assign(1, 0.0);                     // unused by the disc walk
+      nx_cost = ys[i].empty() ? 0 : (int64_t) (sum / (double) ys[i].size() + 1.0);
+      if (mx == 0.0) nx_cost = 0;
+    } else {
+      xs[i] = subgrid(p.a, p.dx);
+      ys[i] = subgrid(p.b, p.dy);
+      nx_cost = (int64_t) xs[i].size();
+    }
     el[i] = element_table(p);
-    const int64_t nx = (int64_t) xs[i].size(), ny = (int64_t) ys[i].size();
+    const int64_t nx = nx_cost, ny = (int64_t) ys[i].size();
     require(L * ny < (1ll << 31) && nx < (1ll << 31), "SIR: element x row count too large");
     SirDev &d = dev[i];
     std::memset(&d, 0, sizeof d);
@@ -488,6 +531,8 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
     d.L = (int32_t) L;
     d.nx = (int32_t) nx;
     d.ny = (int32_t) ny;
+    d.shape = p.shape;
+    d.dx = p.dx;
     int stride = 1;
     if (ny > 64) {
       stride = (int) (ny / 64);
@@ -515,7 +560,7 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
     d.mode = !fits ? SIR_GLOBAL : (d.chunks > 1 ? SIR_LDS_ATOMIC : SIR_LDS_STORE);
     if (fits) max_nt_lds = std::max(max_nt_lds, p.nt);
     cost[i] = (double) Q * (double) nx / d.chunks;
-    table_doubles += xs[i].size() + ys[i].size() + el[i].size();
+    table_doubles += xs[i].size() + ys[i].size() + el[i].size() + rx0[i].size() + rn[i].size();
     levels_.push_back(p.err_level);
   }
 
@@ -546,6 +591,10 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
     tables.insert(tables.end(), ys[i].begin(), ys[i].end());
     d.el = d_tables_ + tables.size();
     tables.insert(tables.end(), el[i].begin(), el[i].end());
+    d.row_x0 = d_tables_ + tables.size();
+    tables.insert(tables.end(), rx0[i].begin(), rx0[i].end());
+    d.row_n = d_tables_ + tables.size();
+    tables.insert(tables.end(), rn[i].begin(), rn[i].end());
     d.status = d_status_ + i;
     d.item_begin = item;
     // A problem with no sub-elements (or no points) contributes no items.

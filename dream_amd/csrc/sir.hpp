@@ -12,7 +12,8 @@ namespace dream {
 // per-element inputs; device pointers for Ro, delay and the output).
 struct SirProblem {
   const double *Ro = nullptr;  int64_t No = 0;          // device
-  double a = 0, b = 0, dx = 0, dy = 0, dt = 0;  int64_t nt = 0;
+  int shape = 0;                                       // SIR_RECT or SIR_CIRC
+  double a = 0, b = 0, R = 0, dx = 0, dy = 0, dt = 0;  int64_t nt = 0;
   int delay_type = 0;  const double *delay = nullptr;   // device
   double v = 1, cp = 1000;
   std::vector<double> G;                               // L x 3 column-major (host)
@@ -30,6 +31,8 @@ struct SirDev {
   const double *delay;  int32_t delay_multi;
   int32_t L;
   const double *xs;  const double *ys;  int32_t nx, ny, row_stride, chunks;
+  // Discs: per-row first x position and sub-element count (doubles).
+  const double *row_x0;  const double *row_n;  double dx;  int32_t shape, pad1;
   int32_t step_e, step_j, step_row, pad0;   // row walk: q += 256 as (e, j, row) steps
   const double *el;                 // 6 x L: gx gy gz foc steer num
   int64_t nt;
@@ -42,6 +45,14 @@ struct SirDev {
 };
 
 enum { SIR_LDS_STORE = 0, SIR_LDS_ATOMIC = 1, SIR_GLOBAL = 2 };
+enum { SIR_RECT = 0, SIR_CIRC = 1 };
+
+// Sub-element rows of a disc of radius R (src/dreamcirc.cc:300-348): ys =
+// -R + dy/2 while ys <= R; in row ys the x walk starts at -sqrt(R*R - ys*ys)
+// + dx/2 and runs while xs <= sqrt(R*R - ys*ys).  Fills ys, the first x and
+// the count of each row.
+void disc_rows(double R, double dx, double dy, std::vector<double> &ys,
+               std::vector<double> &x0, std::vector<double> &count);
 
 // Sub-element centres of one side: s = -w/2 + d/2; while (s <= w/2) s += d
 // (the accumulated sequence of src/dreamrect.cc:286-331).

@@ -137,6 +137,32 @@ int dream_hip_arr_rect_dev(const double *Ro, int64_t No, double a, double b, dou
                            int out_dtype, int err_level, int *sir_err, int64_t *oob_count,
                            void *stream);
 
+/* dream_arr_circ / dreamcirc on the GPU: the same as dream_hip_arr_rect /
+ * dream_hip_rect for circular elements of radius R.  Replace
+ * ArrCirc::dream_arr_circ (include/dream_arr_circ.h, src/dream_arr_circ.cc:
+ * 87-420) and Circ::dreamcirc (include/dreamcirc.h, src/dreamcirc.cc:78-352):
+ * the DR sum walks rows ys = -R + dy/2 (+= dy, while ys <= R) and in each
+ * row xs = -sqrt(R*R - ys*ys) + dx/2 (+= dx, while xs <= sqrt(R*R - ys*ys)),
+ * with the index and amplitude rules of the rectangle.  Host pointers;
+ * FP64 output; alpha must be 0. */
+int dream_hip_arr_circ(double alpha, const double *Ro, int64_t No, double R, double dx,
+                       double dy, double dt, int64_t nt, int delay_type, const double *delay,
+                       double v, double cp, int64_t num_elements, const double *G, int foc_met,
+                       const double *focal, int steer_met, double theta, double phi,
+                       const double *apod, int do_apod, int apod_met, double apod_par,
+                       double *h, int err_level, int *sir_err);
+int dream_hip_circ(double alpha, const double *Ro, int64_t No, double R, double dx, double dy,
+                   double dt, int64_t nt, int delay_type, const double *delay, double v,
+                   double cp, double *h, int err_level, int *sir_err);
+/* Device-pointer variant (conventions of dream_hip_arr_rect_dev). */
+int dream_hip_arr_circ_dev(const double *Ro, int64_t No, double R, double dx, double dy,
+                           double dt, int64_t nt, int delay_type, const double *delay, double v,
+                           double cp, int64_t num_elements, const double *G, int foc_met,
+                           const double *focal, int steer_met, double theta, double phi,
+                           const double *apod, int do_apod, int apod_met, double apod_par,
+                           void *h, int out_dtype, int err_level, int *sir_err,
+                           int64_t *oob_count, void *stream);
+
 /* Per-contribution sample indices of one (observation point, element) pair,
  * computed by the same device code as the SIR kernels: it_out[j] =
  * rint((t_j - delay)/dt) for the j-th sub-element in the reference's (ys
@@ -160,6 +186,8 @@ typedef struct DreamSirProblem {
   const double *apod;  int32_t do_apod;  int32_t apod_met;  double apod_par;
   int32_t err_level;
   void *h;                                   /* nt x No, out_dtype */
+  int32_t shape;                             /* 0 rectangle (a x b), 1 disc (radius R) */
+  double R;                                  /* disc radius [mm] (shape 1) */
 } DreamSirProblem;
 
 /* Build a plan (host preparation, device tables, LPT-ordered work list) for

@@ -395,6 +395,111 @@ int dro_dream_arr_rect(const double *Ro, idx_t No, double a, double b, double dx
   return err;
 }
 
+// ---------------------------------------------------------------------------
+// SIR: discrete-representation sum over a circular disc, src/dreamcirc.cc:286-352.
+// ---------------------------------------------------------------------------
+
+// Rows ys = -R + dy/2, += dy while ys <= R; in row ys the x walk runs from
+// -sqrt(R*R - ys*ys) + dx/2 while xs <= sqrt(R*R - ys*ys).
+static int circ_sum(double xo, double yo, double zo, double R, double dx, double dy,
+                    double dt, idx_t nt, double delay, double v, double cp, double *h,
+                    int level, double weight, long long *oob)
+{
+  int err = E_NONE;
+  const double ds = dx * dy;
+  const double ysmin = -R, ysmax = R;
+  double ys = ysmin + dy / 2.0;
+  while (ys <= ysmax) {
+    const double rxs = std::sqrt(R * R - ys * ys);
+    const double xsmin = -rxs, xsmax = rxs;
+    const double ry = yo - ys;
+    double xs = xsmin + dx / 2.0;
+    while (xs <= xsmax) {
+      const double rx = xo - xs;
+      const double r = std::sqrt(rx * rx + ry * ry + zo * zo);
+      double ai = weight * v * ds / (2 * M_PI * r);
+      ai /= dt;
+      ai *= 1000.0;
+      const double t = r * 1000.0 / cp;
+      const double u = std::rint((t - delay) / dt);
+      const bool inside = (u >= 0.0) && (u < (double) nt);
+      if (inside) {
+        h[(idx_t) u] += ai;
+      } else {
+        if (oob) (*oob)++;
+        err = oob_code(level);
+        if (err == E_OOB) return err;
+      }
+      xs += dx;
+    }
+    ys += dy;
+  }
+  return err;
+}
+
+// Circ::dreamcirc without attenuation (src/dreamcirc.cc:78-272).
+int dro_dreamcirc(const double *Ro, idx_t No, double R, double dx, double dy, double dt,
+                  idx_t nt, int delay_type, const double *delay, double v, double cp, double *h,
+                  int level, int nthreads, long long *oob_total)
+{
+  std::atomic<bool> running(true);
+  std::atomic<long long> oob_sum(0);
+  int err = run_partitioned(No, nthreads, [&](idx_t start, idx_t stop, int &status) {
+    long long oob = 0;
+    for (idx_t n = start; n < stop; n++) {
+      const double dl = (delay_type == D_SINGLE) ? delay[0] : delay[n];
+      int e = circ_sum(Ro[n], Ro[n + No], Ro[n + 2 * No], R, dx, dy, dt, nt, dl, v, cp,
+                       &h[n * nt], level, 1.0, &oob);
+      if (e != E_NONE) status = e;
+      if (e == E_OOB) running = false;
+      if (!running) break;
+    }
+    oob_sum += oob;
+  });
+  if (oob_total) *oob_total = oob_sum.load();
+  return err;
+}
+
+// ArrCirc::dream_arr_circ without attenuation (src/dream_arr_circ.cc:87-236,
+// 300-420): the element loop of dream_arr_rect with the disc sum.
+int dro_dream_arr_circ(const double *Ro, idx_t No, double R, double dx, double dy, double dt,
+                       idx_t nt, int delay_type, const double *delay, double v, double cp,
+                       idx_t L, const double *G, int foc_met, const double *focal,
+                       int steer_met, double theta, double phi, const double *apod, int do_apod,
+                       int apod_met, double apod_par, double *h, int level, int nthreads,
+                       long long *oob_total)
+{
+  const double *gx = G, *gy = G + L, *gz = G + 2 * L;
+  std::atomic<bool> running(true);
+  std::atomic<long long> oob_sum(0);
+  int err = run_partitioned(No, nthreads, [&](idx_t start, idx_t stop, int &status) {
+    long long oob = 0;
+    for (idx_t n = start; n < stop; n++) {
+      const double xo = Ro[n], yo = Ro[n + No], zo = Ro[n + 2 * No];
+      const double dl = (delay_type == D_SINGLE) ? delay[0] : delay[n];
+      double x_max, y_max, r_max;
+      dro_max_dim_arr(gx, gy, L, &x_max, &y_max, &r_max);
+      int e = E_NONE;
+      for (idx_t el = 0; el < L; el++) {
+        double foc = (foc_met == F_UD) ? focal[el]
+                   : dro_focusing(foc_met, focal[0], gx[el], gy[el], x_max, y_max, r_max, cp);
+        double steer = dro_beamsteering(steer_met, theta, phi, gx[el], gy[el], x_max, y_max, cp);
+        double w = do_apod ? dro_apodization(apod_met, el, apod, 1.0, gx[el], gy[el], r_max, apod_par)
+                           : 1.0;
+        e = circ_sum(xo - gx[el], yo - gy[el], zo - gz[el], R, dx, dy, dt, nt,
+                     dl - foc - steer, v, cp, &h[n * nt], level, w, &oob);
+        if (e == E_OOB) break;
+      }
+      if (e != E_NONE) status = e;
+      if (e == E_OOB) running = false;
+      if (!running) break;
+    }
+    oob_sum += oob;
+  });
+  if (oob_total) *oob_total = oob_sum.load();
+  return err;
+}
+
 } // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -77,6 +77,16 @@ def lib():
             ctypes.c_int, _c_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _c_dp,
             ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_dp, ctypes.c_int, ctypes.c_int,
             _c_lp]
+        L.dro_dreamcirc.restype = ctypes.c_int
+        L.dro_dreamcirc.argtypes = [_c_dp, _c_i64] + [ctypes.c_double] * 4 + [
+            _c_i64, ctypes.c_int, _c_dp, ctypes.c_double, ctypes.c_double, _c_dp,
+            ctypes.c_int, ctypes.c_int, _c_lp]
+        L.dro_dream_arr_circ.restype = ctypes.c_int
+        L.dro_dream_arr_circ.argtypes = [_c_dp, _c_i64] + [ctypes.c_double] * 4 + [
+            _c_i64, ctypes.c_int, _c_dp, ctypes.c_double, ctypes.c_double, _c_i64, _c_dp,
+            ctypes.c_int, _c_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _c_dp,
+            ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_dp, ctypes.c_int, ctypes.c_int,
+            _c_lp]
         for name, ptr, scal in (("dro_das_f64", _c_dp, ctypes.c_double),
                                 ("dro_das_f32", _c_fp, ctypes.c_float)):
             f = getattr(L, name)
@@ -199,6 +209,42 @@ def dream_arr_rect(Ro, a, b, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", foc
     err = lib().dro_dream_arr_rect(
         _p(Ro), No, a, b, dx, dy, dt, int(nt), dtype, _p(delay), v, cp, L, _p(G),
         FOC[foc_met], _p(focal), STEER[steer_met], theta, phi, _p(apod_v), int(do_apod),
+        APOD.get(apod_met, 0), apod_par, _p(H), ERR_LEVEL[err_level], int(nthreads),
+        ctypes.byref(oob))
+    return H, err, oob.value
+
+
+def dreamcirc(Ro, R, dx, dy, dt, nt, delay, v, cp, err_level="stop", nthreads=0):
+    """Circ::dreamcirc (no attenuation).  Returns (H, err code, oob count)."""
+    Ro = _f64(np.atleast_2d(Ro))
+    No = Ro.shape[0]
+    delay = _f64(np.atleast_1d(np.asarray(delay, dtype=np.float64).ravel()))
+    H = np.zeros((int(nt), No), dtype=np.float64, order="F")
+    oob = ctypes.c_longlong(0)
+    err = lib().dro_dreamcirc(_p(Ro), No, R, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1,
+                              _p(delay), v, cp, _p(H), ERR_LEVEL[err_level], int(nthreads),
+                              ctypes.byref(oob))
+    return H, err, oob.value
+
+
+def dream_arr_circ(Ro, R, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=0.0,
+                   steer_met="off", theta=0.0, phi=0.0, apod_met="off", apod=None,
+                   apod_par=0.0, err_level="stop", nthreads=0):
+    """ArrCirc::dream_arr_circ (no attenuation).  Returns (H, err code, oob count)."""
+    Ro = _f64(np.atleast_2d(Ro))
+    G = _f64(np.atleast_2d(G))
+    No, L = Ro.shape[0], G.shape[0]
+    delay = _f64(np.atleast_1d(np.asarray(delay, dtype=np.float64).ravel()))
+    focal = _f64(np.atleast_1d(np.asarray(focal, dtype=np.float64).ravel()))
+    if focal.size < L:
+        focal = _f64(np.resize(focal, L))
+    do_apod = apod_met != "off"
+    apod_v = _f64(np.ones(L) if apod is None else np.asarray(apod, dtype=np.float64).ravel())
+    H = np.zeros((int(nt), No), dtype=np.float64, order="F")
+    oob = ctypes.c_longlong(0)
+    err = lib().dro_dream_arr_circ(
+        _p(Ro), No, R, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1, _p(delay), v, cp, L,
+        _p(G), FOC[foc_met], _p(focal), STEER[steer_met], theta, phi, _p(apod_v), int(do_apod),
         APOD.get(apod_met, 0), apod_par, _p(H), ERR_LEVEL[err_level], int(nthreads),
         ctypes.byref(oob))
     return H, err, oob.value
