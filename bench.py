@@ -177,7 +177,44 @@ def bench_sir(steps, warmup):
         contributions_per_s=contrib / (ms_b * 1e-3), in_bounds_contributions=int(inb),
         roofline_frac=(inb * 8 + out_bytes) / (ms_b * 1e-3) / HBM_PEAK,
         per_call_ms=round(ms_p, 3), speedup_vs_per_call=ms_p / ms_b)
+    out.update(bench_annu(steps, warmup))
     return out
+
+
+def _disc_subelements(R, d):
+    """Sub-elements of the disc row walk (rows ys = -R + d/2 while ys <= R)."""
+    import math
+    n, ys = 0, -R + d / 2.0
+    while ys <= R:
+        rx = math.sqrt(max(R * R - ys * ys, 0.0))
+        xs = -rx + d / 2.0
+        while xs <= rx:
+            n += 1
+            xs += d
+        ys += d
+    return n
+
+
+def bench_annu(steps, warmup):
+    """Annular array (dream_arr_annu): the reference's test radii over 4096
+    points, FP64 output.  Contributions = points x sum over discs of their
+    sub-elements."""
+    import torch
+
+    import workloads
+    from dream_amd.torch_ops import arr_annu
+    w = workloads.annu()
+    Ro = torch.tensor(w["Ro"].T.copy(), dtype=torch.float64, device="cuda")
+    No = Ro.shape[1]
+    H = torch.empty((No, w["nt"]), dtype=torch.float64, device="cuda")
+
+    def run():
+        arr_annu(Ro, w["Gr"], w["dx"], w["dy"], w["dt"], w["nt"], w["delay"], w["v"], w["cp"],
+                 foc_met=w["foc_met"], focal=w["focal"], err_level=w["err_level"], out=H)
+    ms = time_launches(run, max(2, steps // 10), 1)
+    contrib = No * sum(_disc_subelements(r, w["dx"]) for r in w["Gr"])
+    return {"annu_4096_points_f64": dict(ms=round(ms, 3), contributions=contrib,
+                                         contributions_per_s=contrib / (ms * 1e-3))}
 
 
 def bench_batches(steps, warmup):

@@ -42,6 +42,22 @@ def grid(func, G, dtype):
     return G
 
 
+def annu_radii(func, G):
+    """check_array_annu (include/arg_parser_python.h:115-127) and the radii the
+    reference reads: the first G.shape[0] values of the column-major data
+    (python/src/py_dream_arr_annu.cc:73-81).  A 1-D array is a column of radii."""
+    a = np.asarray(G, dtype=np.float64)
+    if a.ndim == 0:
+        a = a.reshape(1, 1)
+    elif a.ndim == 1:
+        a = a.reshape(-1, 1)
+    m, n = a.shape[0], a.shape[1]
+    if m != 1 and n != 1 and (n * n) % 2 == 0:
+        fail(func, "requires that array grid arg must be a (number of radii) vector with an odd "
+                   "number of elements!")
+    return np.ascontiguousarray(np.asfortranarray(a).ravel(order="F")[:m])
+
+
 def vector(func, a, n, what, dtype=np.float64):
     """An n-element row or column vector (parse_geometry/parse_sampling/parse_material)."""
     m = as_matrix(a, dtype)

@@ -40,6 +40,8 @@ EXPORTS = (
     "dream_hip_das_uni_create", "dream_hip_das_uni_destroy", "dream_hip_das_uni_shape",
     "dream_hip_das_uni_setup_has_changed", "dream_hip_das_uni", "dream_hip_das_uni_dev",
     "dream_hip_arr_circ", "dream_hip_circ", "dream_hip_arr_circ_dev",
+    "dream_hip_arr_annu", "dream_hip_arr_annu_dev",
+    "dream_hip_arr_cylind", "dream_hip_cylind", "dream_hip_arr_cylind_dev",
 )
 
 _i32, _i64, _dbl = ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -118,6 +120,25 @@ def _declare(L):
     L.dream_hip_arr_circ_dev.argtypes = [_vp, _i64] + [_dbl] * 4 + [_i64, _i32, _vp, _dbl, _dbl] + [
         _i64, _vp, _i32, _vp, _i32, _dbl, _dbl, _vp, _i32, _i32, _dbl, _vp, _i32, _i32, _ip,
         _i64p, _vp]
+    L.dream_hip_arr_cylind.restype = _i32
+    L.dream_hip_arr_cylind.argtypes = [_dbl, _vp, _i64] + [_dbl] * 6 + [_i64, _i32, _vp, _dbl,
+                                                                       _dbl] + [
+        _i64, _vp, _i32, _vp, _i32, _dbl, _dbl, _vp, _i32, _i32, _dbl, _vp, _i32, _ip]
+    L.dream_hip_cylind.restype = _i32
+    L.dream_hip_cylind.argtypes = [_dbl, _vp, _i64] + [_dbl] * 6 + [_i64, _i32, _vp, _dbl, _dbl,
+                                                                   _vp, _i32, _ip]
+    L.dream_hip_arr_cylind_dev.restype = _i32
+    L.dream_hip_arr_cylind_dev.argtypes = [_vp, _i64] + [_dbl] * 6 + [_i64, _i32, _vp, _dbl,
+                                                                     _dbl] + [
+        _i64, _vp, _i32, _vp, _i32, _dbl, _dbl, _vp, _i32, _i32, _dbl, _vp, _i32, _i32, _ip,
+        _i64p, _vp]
+    L.dream_hip_arr_annu.restype = _i32
+    L.dream_hip_arr_annu.argtypes = [_dbl, _vp, _i64] + [_dbl] * 3 + [
+        _i64, _i32, _vp, _dbl, _dbl, _i64, _vp, _i32, _vp, _vp, _i32, _i32, _dbl, _vp, _i32, _ip]
+    L.dream_hip_arr_annu_dev.restype = _i32
+    L.dream_hip_arr_annu_dev.argtypes = [_vp, _i64] + [_dbl] * 3 + [
+        _i64, _i32, _vp, _dbl, _dbl, _i64, _vp, _i32, _vp, _vp, _i32, _i32, _dbl, _vp, _i32,
+        _i32, _ip, _i64p, _vp]
     L.dream_hip_rect_indices.restype = _i32
     L.dream_hip_rect_indices.argtypes = [_dbl] * 10 + [_i64p, _i64, _i64p]
     L.dream_hip_sir_plan_create.restype = _i32

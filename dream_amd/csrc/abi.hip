@@ -147,6 +147,22 @@ SirProblem make_circ_problem(const double *Ro, int64_t No, double R, double dx, 
   return p;
 }
 
+// A cylindrically curved problem: sides a, b and curvature radius Rcurv.
+SirProblem make_cylind_problem(const double *Ro, int64_t No, double a, double b, double Rcurv,
+                               double dx, double dy, double dt, int64_t nt, int delay_type,
+                               const double *delay, double v, double cp, int64_t L,
+                               const double *G, int foc_met, const double *focal, int steer_met,
+                               double theta, double phi, const double *apod, int do_apod,
+                               int apod_met, double apod_par, void *h, int err_level)
+{
+  SirProblem p = make_problem(Ro, No, a, b, dx, dy, dt, nt, delay_type, delay, v, cp, L, G,
+                              foc_met, focal, steer_met, theta, phi, apod, do_apod, apod_met,
+                              apod_par, h, err_level);
+  p.shape = SIR_CYLIND;
+  p.R = Rcurv;
+  return p;
+}
+
 // One SIR problem on the device: run a single-problem plan, then read its status.
 void run_single(const SirProblem &p, int out_dtype, hipStream_t s, int *sir_err,
                 int64_t *oob_count)
@@ -314,6 +330,105 @@ int dream_hip_circ(double alpha, const double *Ro, int64_t No, double R, double 
                             &focal, 0, 0.0, 0.0, nullptr, 0, 0, 0.0, h, err_level, sir_err);
 }
 
+int dream_hip_arr_cylind_dev(const double *Ro, int64_t No, double a, double b, double Rcurv,
+                             double dx, double dy, double dt, int64_t nt, int delay_type,
+                             const double *delay, double v, double cp, int64_t num_elements,
+                             const double *G, int foc_met, const double *focal, int steer_met,
+                             double theta, double phi, const double *apod, int do_apod,
+                             int apod_met, double apod_par, void *h, int out_dtype,
+                             int err_level, int *sir_err, int64_t *oob_count, void *stream)
+{
+  return guarded([&]() {
+    run_single(make_cylind_problem(Ro, No, a, b, Rcurv, dx, dy, dt, nt, delay_type, delay, v, cp,
+                                   num_elements, G, foc_met, focal, steer_met, theta, phi, apod,
+                                   do_apod, apod_met, apod_par, h, err_level),
+               out_dtype, (hipStream_t) stream, sir_err, oob_count);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_arr_cylind(double alpha, const double *Ro, int64_t No, double a, double b,
+                         double Rcurv, double dx, double dy, double dt, int64_t nt,
+                         int delay_type, const double *delay, double v, double cp,
+                         int64_t num_elements, const double *G, int foc_met,
+                         const double *focal, int steer_met, double theta, double phi,
+                         const double *apod, int do_apod, int apod_met, double apod_par,
+                         double *h, int err_level, int *sir_err)
+{
+  return guarded([&]() {
+    return run_host_sir(make_cylind_problem(nullptr, No, a, b, Rcurv, dx, dy, dt, nt, delay_type,
+                                            nullptr, v, cp, num_elements, G, foc_met, focal,
+                                            steer_met, theta, phi, apod, do_apod, apod_met,
+                                            apod_par, nullptr, err_level),
+                        alpha, Ro, delay, h, sir_err);
+  });
+}
+
+int dream_hip_cylind(double alpha, const double *Ro, int64_t No, double a, double b, double Rcurv,
+                     double dx, double dy, double dt, int64_t nt, int delay_type,
+                     const double *delay, double v, double cp, double *h, int err_level,
+                     int *sir_err)
+{
+  const double G[3] = {0.0, 0.0, 0.0};
+  const double focal = 0.0;
+  return dream_hip_arr_cylind(alpha, Ro, No, a, b, Rcurv, dx, dy, dt, nt, delay_type, delay, v,
+                              cp, 1, G, 0, &focal, 0, 0.0, 0.0, nullptr, 0, 0, 0.0, h, err_level,
+                              sir_err);
+}
+
+int dream_hip_arr_annu_dev(const double *Ro, int64_t No, double dx, double dy, double dt,
+                           int64_t nt, int delay_type, const double *delay, double v, double cp,
+                           int64_t num_radii, const double *Gr, int foc_met, const double *focal,
+                           const double *apod, int do_apod, int apod_met, double apod_par,
+                           void *h, int out_dtype, int err_level, int *sir_err,
+                           int64_t *oob_count, void *stream)
+{
+  return guarded([&]() {
+    require(err_level >= 0 && err_level <= 4, "dream_arr_annu: unknown error level");
+    require(No == 0 || (Ro && delay && h), "dream_arr_annu: null argument");
+    AnnuProblem a;
+    a.Ro = Ro;  a.No = No;
+    a.dx = dx;  a.dy = dy;  a.dt = dt;  a.nt = nt;
+    a.delay_type = delay_type;  a.delay = delay;
+    a.v = v;  a.cp = cp;
+    a.num_radii = num_radii;  a.Gr = Gr;
+    a.foc_met = foc_met;  a.focal = focal;
+    a.apod = apod;  a.do_apod = do_apod;  a.apod_met = apod_met;  a.apod_par = apod_par;
+    a.err_level = err_level;
+    a.h = h;
+    run_arr_annu(a, out_dtype, (hipStream_t) stream, sir_err, oob_count);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_arr_annu(double alpha, const double *Ro, int64_t No, double dx, double dy,
+                       double dt, int64_t nt, int delay_type, const double *delay, double v,
+                       double cp, int64_t num_radii, const double *Gr, int foc_met,
+                       const double *focal, const double *apod, int do_apod, int apod_met,
+                       double apod_par, double *h, int err_level, int *sir_err)
+{
+  return guarded([&]() {
+    if (alpha > 2.220446049250313e-16)
+      throw Error(DREAM_HIP_EUNSUPPORTED,
+                  "dream_arr_annu: attenuation (alpha > 0) is not on the GPU path");
+    require(No >= 0 && nt >= 1, "dream_arr_annu: need No >= 0 and nt >= 1");
+    require(No == 0 || (Ro && delay && h), "dream_arr_annu: null argument");
+    hipStream_t s = lib_stream();
+    const int64_t nd = delay_type ? No : 1;
+    DevBuf dRo(Ro, (size_t) No * 3 * 8, s), dD(delay, (size_t) nd * 8, s);
+    DevBuf dH((size_t) No * nt * 8);
+    int code = 0;
+    int rc = dream_hip_arr_annu_dev((const double *) dRo.p, No, dx, dy, dt, nt, delay_type,
+                                    (const double *) dD.p, v, cp, num_radii, Gr, foc_met, focal,
+                                    apod, do_apod, apod_met, apod_par, dH.p, DREAM_HIP_F64,
+                                    err_level, &code, nullptr, s);
+    if (rc != DREAM_HIP_OK) throw Error(rc, g_last_error);
+    if (No) DREAM_HIP_CHECK(hipMemcpy(h, dH.p, (size_t) No * nt * 8, hipMemcpyDeviceToHost));
+    if (sir_err) *sir_err = code;
+    return DREAM_HIP_OK;
+  });
+}
+
 int dream_hip_rect_indices(double xo, double yo, double zo, double a, double b, double dx,
                            double dy, double dt, double delay, double cp, int64_t *it_out,
                            int64_t cap, int64_t *count)
@@ -335,7 +450,8 @@ int dream_hip_sir_plan_create(const DreamSirProblem *problems, int32_t n, int ou
     probs.reserve((size_t) n);
     for (int32_t i = 0; i < n; i++) {
       const DreamSirProblem &q = problems[i];
-      require(q.shape == SIR_RECT || q.shape == SIR_CIRC, "sir_plan_create: unknown shape");
+      require(q.shape == SIR_RECT || q.shape == SIR_CIRC || q.shape == SIR_CYLIND,
+              "sir_plan_create: unknown shape");
       probs.push_back(make_problem(q.Ro, q.No, q.a, q.b, q.dx, q.dy, q.dt, q.nt, q.delay_type,
                                    q.delay, q.v, q.cp, q.num_elements, q.G, q.foc_met, q.focal,
                                    q.steer_met, q.theta, q.phi, q.apod, q.do_apod, q.apod_met,

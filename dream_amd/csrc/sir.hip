@@ -153,7 +153,10 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
     const double d = (dlay - el[3 * L + e]) - el[4 * L + e];
     const double ry = ye - ys[row];
     const double ryy = ry * ry;
-    const double rzz = ze * ze;
+    // Cylinders: the row's depth on the curved surface (ze - 0.0 == ze for
+    // flat elements).
+    const double rz = ze - (P.shape == SIR_CYLIND ? ((gdouble *) P.zrow)[row] : 0.0);
+    const double rzz = rz * rz;
     const double c2 = d * P.inv_dt;
     const double lim = 0.5 - kBoundScale * (ntd + 2.0 + __builtin_fabs(c2));
     const double amp = el[5 * L + e] * P.amp;
@@ -351,6 +354,29 @@ void disc_rows(double R, double dx, double dy, std::vector<double> &ys, std::vec
   }
 }
 
+double cylinder_rows(double b, double Rc, double dy, std::vector<double> &ys,
+                     std::vector<double> &zrow)
+{
+  ys.clear();
+  zrow.clear();
+  const double R = std::fabs(Rc);
+  const double hb = b / 2.0;
+  const double z_R = R - std::sqrt(Rc * Rc - hb * hb);
+  const double span = 2.0 * std::atan(hb / (R - z_R));
+  const double ph_min = -span / 2.0, ph_max = span / 2.0;
+  const double dphi = std::asin(dy / R);
+  double ph = ph_min + dphi / 2.0;
+  double y = R * std::sin(ph);
+  while (ph <= ph_max) {            // false throughout when b > 2|Rc| (NaN angles)
+    const double zs = R - std::sqrt(R * R - y * y);
+    ys.push_back(y);
+    zrow.push_back(Rc >= 0.0 ? zs : -zs);
+    ph += dphi;
+    y = R * std::sin(ph);
+  }
+  return dphi;
+}
+
 std::vector<double> subgrid(double w, double d)
 {
   std::vector<double> v;
@@ -456,7 +482,11 @@ std::vector<double> element_table(const SirProblem &p)
   const double *gx = p.G.data(), *gy = gx + L, *gz = gy + L;
   double x_max, y_max, r_max;
   aperture_extent(gx, gy, L, x_max, y_max, r_max);
-  const double ds = p.dx * p.dy;
+  double ds = p.dx * p.dy;
+  if (p.shape == SIR_CYLIND) {
+    std::vector<double> ys, zr;
+    ds = std::fabs(p.R) * p.dx * cylinder_rows(p.b, p.R, p.dy, ys, zr);
+  }
   std::vector<double> t((size_t) (6 * L));
   for (int64_t e = 0; e < L; e++) {
     const double foc = (p.foc_met == 7) ? p.focal[(size_t) e]
@@ -493,7 +523,7 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
   for (auto &p : problems) total_no += p.No;
 
   // Host tables and per-problem shape.
-  std::vector<std::vector<double>> xs(P), ys(P), el(P), rx0(P), rn(P);
+  std::vector<std::vector<double>> xs(P), ys(P), el(P), rx0(P), rn(P), zr(P);
   std::vector<SirDev> dev(P);
   std::vector<double> cost(P, 0.0);
   size_t table_doubles = 0;
@@ -505,7 +535,8 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
     require(L >= 1, "SIR: need at least one element");
     require(p.dx > 0 && p.dy > 0 && p.dt > 0, "SIR: dx, dy and dt must be positive");
     require(p.No == 0 || (p.Ro && p.delay && p.h), "SIR: null Ro, delay or h");
-    require(p.shape == SIR_RECT || p.shape == SIR_CIRC, "SIR: unknown element shape");
+    require(p.shape == SIR_RECT || p.shape == SIR_CIRC || p.shape == SIR_CYLIND,
+            "SIR: unknown element shape");
     int64_t nx_cost;
     if (p.shape == SIR_CIRC) {
       disc_rows(p.R, p.dx, p.dy, ys[i], rx0[i], rn[i]);
@@ -514,6 +545,10 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
       xs[i].assign(1, 0.0);                     // unused by the disc walk
       nx_cost = ys[i].empty() ? 0 : (int64_t) (sum / (double) ys[i].size() + 1.0);
       if (mx == 0.0) nx_cost = 0;
+    } else if (p.shape == SIR_CYLIND) {
+      xs[i] = subgrid(p.a, p.dx);
+      cylinder_rows(p.b, p.R, p.dy, ys[i], zr[i]);
+      nx_cost = (int64_t) xs[i].size();
     } else {
       xs[i] = subgrid(p.a, p.dx);
       ys[i] = subgrid(p.b, p.dy);
@@ -539,9 +574,12 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
       while (gcd_int(stride, (int) ny) != 1) stride++;
     }
     d.row_stride = stride;
-    d.step_e = (int32_t) (kThreads / ny);
-    d.step_j = (int32_t) (kThreads % ny);
-    d.step_row = (int32_t) (((int64_t) d.step_j * stride) % ny);
+    // An element with no rows (a side or radius below half a step) has no
+    // work items; its walk steps are never used.
+    const int64_t ny_w = std::max<int64_t>(ny, 1);
+    d.step_e = (int32_t) (kThreads / ny_w);
+    d.step_j = (int32_t) (kThreads % ny_w);
+    d.step_row = (int32_t) (((int64_t) d.step_j * stride) % ny_w);
     d.nt = p.nt;
     d.cp = p.cp;
     d.dt = p.dt;
@@ -560,7 +598,8 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
     d.mode = !fits ? SIR_GLOBAL : (d.chunks > 1 ? SIR_LDS_ATOMIC : SIR_LDS_STORE);
     if (fits) max_nt_lds = std::max(max_nt_lds, p.nt);
     cost[i] = (double) Q * (double) nx / d.chunks;
-    table_doubles += xs[i].size() + ys[i].size() + el[i].size() + rx0[i].size() + rn[i].size();
+    table_doubles += xs[i].size() + ys[i].size() + el[i].size() + rx0[i].size() + rn[i].size()
+                   + zr[i].size();
     levels_.push_back(p.err_level);
   }
 
@@ -595,6 +634,8 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
     tables.insert(tables.end(), rx0[i].begin(), rx0[i].end());
     d.row_n = d_tables_ + tables.size();
     tables.insert(tables.end(), rn[i].begin(), rn[i].end());
+    d.zrow = d_tables_ + tables.size();
+    tables.insert(tables.end(), zr[i].begin(), zr[i].end());
     d.status = d_status_ + i;
     d.item_begin = item;
     // A problem with no sub-elements (or no points) contributes no items.

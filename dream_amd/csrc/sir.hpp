@@ -12,7 +12,9 @@ namespace dream {
 // per-element inputs; device pointers for Ro, delay and the output).
 struct SirProblem {
   const double *Ro = nullptr;  int64_t No = 0;          // device
-  int shape = 0;                                       // SIR_RECT or SIR_CIRC
+  int shape = 0;                                       // SIR_RECT, SIR_CIRC or SIR_CYLIND
+  // Rectangles and cylinders: sides a (x), b (y); discs: radius R;
+  // cylinders: curvature radius R (> 0 focused, < 0 defocused).
   double a = 0, b = 0, R = 0, dx = 0, dy = 0, dt = 0;  int64_t nt = 0;
   int delay_type = 0;  const double *delay = nullptr;   // device
   double v = 1, cp = 1000;
@@ -33,6 +35,8 @@ struct SirDev {
   const double *xs;  const double *ys;  int32_t nx, ny, row_stride, chunks;
   // Discs: per-row first x position and sub-element count (doubles).
   const double *row_x0;  const double *row_n;  double dx;  int32_t shape, pad1;
+  // Cylinders: per-row z offset of the curved surface (rz = z - zrow).
+  const double *zrow;
   int32_t step_e, step_j, step_row, pad0;   // row walk: q += 256 as (e, j, row) steps
   const double *el;                 // 6 x L: gx gy gz foc steer num
   int64_t nt;
@@ -45,7 +49,7 @@ struct SirDev {
 };
 
 enum { SIR_LDS_STORE = 0, SIR_LDS_ATOMIC = 1, SIR_GLOBAL = 2 };
-enum { SIR_RECT = 0, SIR_CIRC = 1 };
+enum { SIR_RECT = 0, SIR_CIRC = 1, SIR_CYLIND = 2 };
 
 // Sub-element rows of a disc of radius R (src/dreamcirc.cc:300-348): ys =
 // -R + dy/2 while ys <= R; in row ys the x walk starts at -sqrt(R*R - ys*ys)
@@ -53,6 +57,16 @@ enum { SIR_RECT = 0, SIR_CIRC = 1 };
 // the count of each row.
 void disc_rows(double R, double dx, double dy, std::vector<double> &ys,
                std::vector<double> &x0, std::vector<double> &count);
+
+// Sub-element rows of a cylindrically curved element of y-size b and
+// curvature radius Rc (src/dreamcylind.cc:288-350): the rows are angles phi
+// from -phi_max + dphi/2 in steps dphi = asin(dy/|Rc|) while phi <= phi_max,
+// phi_max = atan((b/2)/(|Rc| - z_Rc)), z_Rc = |Rc| - sqrt(Rc^2 - (b/2)^2);
+// row k sits at ys = |Rc| sin(phi) and at depth zs = |Rc| - sqrt(Rc^2 - ys^2),
+// stored as zrow = zs (focused, Rc >= 0: rz = z - zs) or -zs (defocused:
+// rz = z + zs).  Returns dphi (the surface element is |Rc| dx dphi).
+double cylinder_rows(double b, double Rc, double dy, std::vector<double> &ys,
+                     std::vector<double> &zrow);
 
 // Sub-element centres of one side: s = -w/2 + d/2; while (s <= w/2) s += d
 // (the accumulated sequence of src/dreamrect.cc:286-331).
@@ -99,6 +113,25 @@ private:
   hipStream_t last_stream_ = nullptr;
   bool ran_ = false;
 };
+
+// One dream_arr_annu call (src/dream_arr_annu.cc:514-634): Gr, focal and
+// apod are host arrays, Ro, delay and h device pointers.
+struct AnnuProblem {
+  const double *Ro = nullptr;  int64_t No = 0;
+  double dx = 0, dy = 0, dt = 0;  int64_t nt = 0;
+  int delay_type = 0;  const double *delay = nullptr;
+  double v = 1, cp = 1000;
+  int64_t num_radii = 0;  const double *Gr = nullptr;
+  int foc_met = 0;  const double *focal = nullptr;
+  const double *apod = nullptr;  int do_apod = 0, apod_met = 0;  double apod_par = 0;
+  int err_level = L_STOP;
+  void *h = nullptr;   // nt x No, FP64 or FP32
+};
+
+// Annular array response: every disc through one SirPlan, then the ring
+// superposition.  Synchronises `s`; *sir_err gets the reference's SIRError.
+void run_arr_annu(const AnnuProblem &a, int out_dtype, hipStream_t s, int *sir_err,
+                  int64_t *oob_count);
 
 // Device parity helper: per-contribution sample indices of one
 // (point, element) pair from the same device index routine.

@@ -254,11 +254,72 @@ def arr_circ(Ro, R, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=0.0,
     return H, code.value, oob.value
 
 
+def arr_cylind(Ro, a, b, Rcurv, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=0.0,
+               steer_met="off", theta=0.0, phi=0.0, apod_met="off", apod=None, apod_par=0.0,
+               err_level="ignore", out_dtype=torch.float64, out=None):
+    """dream_arr_cylind on device tensors (cylindrically curved elements a x b,
+    curvature radius Rcurv); the conventions of arr_rect.  Returns (H (No, nt),
+    SIRError code, out-of-bounds contribution count)."""
+    _lib.require_device()
+    assert Ro.dtype == torch.float64 and Ro.is_contiguous() and Ro.shape[0] == 3
+    No = Ro.shape[1]
+    Gf = np.asfortranarray(np.asarray(G, dtype=np.float64).reshape(-1, 3))
+    L = Gf.shape[0]
+    if torch.is_tensor(delay) and delay.numel() > 1:
+        d, dtype = delay.to(dtype=torch.float64).contiguous(), 1
+    else:
+        d, dtype = torch.tensor([float(delay)], dtype=torch.float64, device=Ro.device), 0
+    focal_v = _host_f64(np.atleast_1d(focal))
+    apod_v = _host_f64(np.ones(L) if apod is None else apod)
+    H = out if out is not None else torch.empty((No, int(nt)), dtype=out_dtype, device=Ro.device)
+    code = ctypes.c_int(0)
+    oob = ctypes.c_int64(0)
+    _lib.check(_lib.lib().dream_hip_arr_cylind_dev(
+        _ptr(Ro), No, a, b, Rcurv, dx, dy, dt, int(nt), dtype, _ptr(d), v, cp, L, Gf.ctypes.data,
+        _lib.FOC_MET[foc_met], focal_v.ctypes.data, _lib.STEER_MET[steer_met], theta, phi,
+        apod_v.ctypes.data, 0 if apod_met == "off" else 1, _lib.APOD_MET.get(apod_met, 0),
+        apod_par, _ptr(H), _lib.F64 if out_dtype == torch.float64 else _lib.F32,
+        _lib.ERR_LEVEL[err_level], ctypes.byref(code), ctypes.byref(oob), _stream()))
+    return H, code.value, oob.value
+
+
+def arr_annu(Ro, Gr, dx, dy, dt, nt, delay, v, cp, foc_met="off", focal=0.0,
+             apod_met="off", apod=None, apod_par=0.0, err_level="ignore",
+             out_dtype=torch.float64, out=None):
+    """dream_arr_annu on device tensors: Ro is (3, No) float64, Gr the host
+    vector of radii (centre disc, then inner/outer per ring).  Returns
+    (H (No, nt), SIRError code, out-of-bounds contribution count)."""
+    _lib.require_device()
+    assert Ro.dtype == torch.float64 and Ro.is_contiguous() and Ro.shape[0] == 3
+    No = Ro.shape[1]
+    Gr_v = _host_f64(np.asarray(Gr, dtype=np.float64).ravel())
+    nr = Gr_v.shape[0]
+    if torch.is_tensor(delay) and delay.numel() > 1:
+        d, dtype = delay.to(dtype=torch.float64).contiguous(), 1
+    else:
+        d, dtype = torch.tensor([float(delay)], dtype=torch.float64, device=Ro.device), 0
+    focal_v = _host_f64(np.atleast_1d(focal))
+    if focal_v.shape[0] < nr:
+        focal_v = _host_f64(np.resize(focal_v, nr))
+    apod_v = _host_f64(np.ones(nr) if apod is None else apod)
+    H = out if out is not None else torch.empty((No, int(nt)), dtype=out_dtype, device=Ro.device)
+    code = ctypes.c_int(0)
+    oob = ctypes.c_int64(0)
+    _lib.check(_lib.lib().dream_hip_arr_annu_dev(
+        _ptr(Ro), No, dx, dy, dt, int(nt), dtype, _ptr(d), v, cp, nr, Gr_v.ctypes.data,
+        _lib.FOC_MET[foc_met], focal_v.ctypes.data, apod_v.ctypes.data,
+        0 if apod_met == "off" else 1, _lib.APOD_MET.get(apod_met, 0), apod_par, _ptr(H),
+        _lib.F64 if out_dtype == torch.float64 else _lib.F32, _lib.ERR_LEVEL[err_level],
+        ctypes.byref(code), ctypes.byref(oob), _stream()))
+    return H, code.value, oob.value
+
+
 class SirBatch:
     """Many uneven dream_arr_rect problems in one persistent launch.
 
     problems: list of dicts with keys Ro ((3, No) float64 device tensor), a, b
-    (rectangles) or shape="circ" and R (discs), dx, dy, dt, nt, delay (float
+    (rectangles), shape="circ" and R (discs) or shape="cylind", a, b and Rcurv
+    (cylindrically curved elements), dx, dy, dt, nt, delay (float
     or (No,) device tensor), v, cp, G ((L, 3) host array), and optionally
     foc_met, focal, steer_met, theta, phi, apod_met, apod, apod_par,
     err_level.  Outputs H[i] are (No_i, nt_i)
@@ -303,8 +364,8 @@ class SirBatch:
             q.apod_met, q.apod_par = _lib.APOD_MET.get(apod_met, 0), p.get("apod_par", 0.0)
             q.err_level = _lib.ERR_LEVEL[p.get("err_level", "ignore")]
             q.h = H.data_ptr()
-            q.shape = 1 if p.get("shape", "rect") == "circ" else 0
-            q.R = float(p.get("R", 0.0))
+            q.shape = {"rect": 0, "circ": 1, "cylind": 2}[p.get("shape", "rect")]
+            q.R = float(p.get("Rcurv", 0.0) if q.shape == 2 else p.get("R", 0.0))
         self.plan = ctypes.c_void_p(None)
         _lib.check(_lib.lib().dream_hip_sir_plan_create(arr, self.n,
                                                         _lib.F64 if out_dtype == torch.float64

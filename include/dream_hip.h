@@ -163,6 +163,67 @@ int dream_hip_arr_circ_dev(const double *Ro, int64_t No, double R, double dx, do
                            void *h, int out_dtype, int err_level, int *sir_err,
                            int64_t *oob_count, void *stream);
 
+/* dream_arr_cylind / dreamcylind on the GPU: the same as dream_hip_arr_rect /
+ * dream_hip_rect for cylindrically curved elements of x-size a, y-size b
+ * and curvature radius Rcurv (> 0 focused/concave, < 0 defocused/convex).
+ * Replace ArrCylind::dream_arr_cylind (include/dream_arr_cylind.h,
+ * src/dream_arr_cylind.cc:90-437) and Cylind::dreamcylind
+ * (include/dreamcylind.h, src/dreamcylind.cc:78-353): the DR sum walks
+ * angle rows phi = -phi_max + dphi/2 (+= dphi = asin(dy/|Rcurv|), while
+ * phi <= phi_max) at ys = |Rcurv| sin(phi), depth zs = |Rcurv| -
+ * sqrt(Rcurv^2 - ys^2) (rz = z - zs focused, z + zs defocused), and in each
+ * row the rectangle's x walk; the surface element is |Rcurv| dx dphi.  With
+ * b > 2|Rcurv| the reference's angles are not numbers and the response is
+ * zero.  Host pointers; FP64 output; alpha must be 0. */
+int dream_hip_arr_cylind(double alpha, const double *Ro, int64_t No, double a, double b,
+                         double Rcurv, double dx, double dy, double dt, int64_t nt,
+                         int delay_type, const double *delay, double v, double cp,
+                         int64_t num_elements, const double *G, int foc_met,
+                         const double *focal, int steer_met, double theta, double phi,
+                         const double *apod, int do_apod, int apod_met, double apod_par,
+                         double *h, int err_level, int *sir_err);
+int dream_hip_cylind(double alpha, const double *Ro, int64_t No, double a, double b, double Rcurv,
+                     double dx, double dy, double dt, int64_t nt, int delay_type,
+                     const double *delay, double v, double cp, double *h, int err_level,
+                     int *sir_err);
+/* Device-pointer variant (conventions of dream_hip_arr_rect_dev). */
+int dream_hip_arr_cylind_dev(const double *Ro, int64_t No, double a, double b, double Rcurv,
+                             double dx, double dy, double dt, int64_t nt, int delay_type,
+                             const double *delay, double v, double cp, int64_t num_elements,
+                             const double *G, int foc_met, const double *focal, int steer_met,
+                             double theta, double phi, const double *apod, int do_apod,
+                             int apod_met, double apod_par, void *h, int out_dtype,
+                             int err_level, int *sir_err, int64_t *oob_count, void *stream);
+
+/* dream_arr_annu on the GPU.  Replaces ArrAnnu::dream_arr_annu
+ * (include/dream_arr_annu.h:40-50, src/dream_arr_annu.cc:514-634).  Gr holds
+ * num_radii radii: the centre disc, then an (inner, outer) pair per ring, so
+ * there are (num_radii + 1)/2 rings.  Ring n's response is the disc response
+ * of radius Gr[2n] minus that of Gr[2n-1] (ring 0: disc 0, left zero when
+ * Gr[0] <= DBL_EPSILON), moved later by rint(foc_n/dt) + 1 samples and cut at
+ * nt, times its weight.  Focusing x / y / xy / x+y use the spherical law
+ * (sqrt(r_max^2 + focal[0]^2) - sqrt(r_n^2 + focal[0]^2)) * 1000 / cp about
+ * the ring's mean radius r_n; 'ud' takes focal[n] in us.  Apodization uses
+ * r_n / r_max; 'ud' takes apod[n]; Hann and Hamming give 1.  Gr, focal and
+ * apod are host arrays (focal and apod: num_radii values for 'ud', of which
+ * the first (num_radii+1)/2 are used).  *sir_err: E_OOB if any disc of any
+ * point stopped; otherwise the status of the last disc.  Host pointers for
+ * Ro, delay and h; FP64 output; alpha must be 0. */
+int dream_hip_arr_annu(double alpha, const double *Ro, int64_t No, double dx, double dy,
+                       double dt, int64_t nt, int delay_type, const double *delay, double v,
+                       double cp, int64_t num_radii, const double *Gr, int foc_met,
+                       const double *focal, const double *apod, int do_apod, int apod_met,
+                       double apod_par, double *h, int err_level, int *sir_err);
+/* Device-pointer variant (conventions of dream_hip_arr_rect_dev; Gr, focal
+ * and apod stay on the host).  Allocates num_radii x No x nt doubles of
+ * device scratch for the disc responses during the call. */
+int dream_hip_arr_annu_dev(const double *Ro, int64_t No, double dx, double dy, double dt,
+                           int64_t nt, int delay_type, const double *delay, double v, double cp,
+                           int64_t num_radii, const double *Gr, int foc_met, const double *focal,
+                           const double *apod, int do_apod, int apod_met, double apod_par,
+                           void *h, int out_dtype, int err_level, int *sir_err,
+                           int64_t *oob_count, void *stream);
+
 /* Per-contribution sample indices of one (observation point, element) pair,
  * computed by the same device code as the SIR kernels: it_out[j] =
  * rint((t_j - delay)/dt) for the j-th sub-element in the reference's (ys
@@ -186,8 +247,9 @@ typedef struct DreamSirProblem {
   const double *apod;  int32_t do_apod;  int32_t apod_met;  double apod_par;
   int32_t err_level;
   void *h;                                   /* nt x No, out_dtype */
-  int32_t shape;                             /* 0 rectangle (a x b), 1 disc (radius R) */
-  double R;                                  /* disc radius [mm] (shape 1) */
+  int32_t shape;                             /* 0 rectangle (a x b), 1 disc (radius R),
+                                                2 cylinder (a x b, curvature radius R) */
+  double R;                                  /* disc radius / curvature radius [mm] */
 } DreamSirProblem;
 
 /* Build a plan (host preparation, device tables, LPT-ordered work list) for

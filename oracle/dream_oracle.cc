@@ -12,6 +12,10 @@
 //        ArrRect driver            src/dream_arr_rect.cc:90-244, 310-433
 //        array helpers             src/arr_functions.cc:60-262
 //        error mapping             src/dream_error.cc:40-76
+//        dreamcirc / dream_arr_circ src/dreamcirc.cc:286-352, src/dream_arr_circ.cc
+//        dream_arr_annu            src/dream_arr_annu.cc:186-254, 361-505, 514-634
+//        dreamcylind / dream_arr_cylind src/dreamcylind.cc:275-353, 442-517,
+//                                  src/dream_arr_cylind.cc:197-250
 //   DAS  DAS<T>::das / smp_das     extra_src/das.cc:103-298
 //        saft/tfm/rca kernels      extra_src/das.cc:306-626
 // plus this project's own extensions (linear interpolation, apodization and
@@ -32,6 +36,13 @@
 //     expression rint(((r*1000)/cp - delay)/dt) is fixed by reading
 //     src/dreamrect.cc:303-311 and is not executable-pinned (no compiled
 //     reference): "parity partially pinned" for that one detail.
+//   * dream_arr_annu (no golden vectors in the reference): pinned by reduction
+//     to the disc sum (one radius = dreamcirc one sample later; a ring =
+//     outer disc - inner disc), the analytic on-axis ring SIR (plateau v*cp
+//     between the inner and outer arrival times, window placement under 'ud'
+//     delays and weights), alignment of the ring pulses at the focal depth
+//     under x focusing, window values of the apodizations and the error-code
+//     rule (tests/test_oracle_annu.py).
 //   * Extensions (linear interpolation, apodization, adjoint): pinned by the
 //     dot-product identity <A y, g> = <y, A^T g>, central finite
 //     differences and reduction to the reference path (linear -> nearest at
@@ -40,6 +51,7 @@
 // Build: oracle/Makefile (g++ -O3 -ffp-contract=off, no -march flags, the
 // same as the reference's Release build).
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -489,6 +501,239 @@ int dro_dream_arr_circ(const double *Ro, idx_t No, double R, double dx, double d
         e = circ_sum(xo - gx[el], yo - gy[el], zo - gz[el], R, dx, dy, dt, nt,
                      dl - foc - steer, v, cp, &h[n * nt], level, w, &oob);
         if (e == E_OOB) break;
+      }
+      if (e != E_NONE) status = e;
+      if (e == E_OOB) running = false;
+      if (!running) break;
+    }
+    oob_sum += oob;
+  });
+  if (oob_total) *oob_total = oob_sum.load();
+  return err;
+}
+
+// ---------------------------------------------------------------------------
+// Cylindrically curved elements, src/dreamcylind.cc:275-353 (DR sum) with
+// the surface point of cylind_f / cylind_d (:442-517; their angle checks are
+// disabled in the reference, so du = 1).
+// ---------------------------------------------------------------------------
+
+// Rows are angles phi = -phi_max + dphi/2, += dphi (dphi = asin(dy/|Rc|))
+// while phi <= phi_max; a row sits at ys = |Rc| sin(phi) and depth zs = |Rc|
+// - sqrt(Rc^2 - ys^2) below the vertex plane (z - zs focused, z + zs
+// defocused); the x walk is the rectangle's.  b > 2|Rc| makes the angles
+// NaN: no rows, a zero response.
+static int cylind_sum(double xo, double yo, double zo, double a, double b, double Rc,
+                      double dx, double dy, double dt, idx_t nt, double delay, double v,
+                      double cp, double *h, int level, double weight, long long *oob)
+{
+  int err = E_NONE;
+  const double R = std::fabs(Rc);
+  const double hb = b / 2.0;
+  const double z_R = R - std::sqrt(Rc * Rc - hb * hb);
+  const double span = 2.0 * std::atan(hb / (R - z_R));
+  const double dphi = std::asin(dy / R);
+  const double ds = R * dx * dphi;
+  double ph = -span / 2.0 + dphi / 2.0;
+  double ys = R * std::sin(ph);
+  while (ph <= span / 2.0) {
+    const double zs = R - std::sqrt(R * R - ys * ys);
+    const double rz = (Rc >= 0.0) ? zo - zs : zo + zs;
+    double xs = -a / 2.0 + dx / 2.0;
+    while (xs <= a / 2.0) {
+      const double rx = xo - xs, ry = yo - ys;
+      const double r = std::sqrt(rx * rx + ry * ry + rz * rz);
+      double ai = weight * v * ds * 1.0 / (2.0 * M_PI * r);
+      ai /= dt;
+      ai *= 1.0e3;
+      const double t = r * 1.0e3 / cp;
+      const double u = std::rint((t - delay) / dt);
+      if (u >= 0.0 && u < (double) nt) {
+        h[(idx_t) u] += ai;
+      } else {
+        if (oob) (*oob)++;
+        err = oob_code(level);
+        if (err == E_OOB) return err;
+      }
+      xs += dx;
+    }
+    ph += dphi;
+    ys = R * std::sin(ph);
+  }
+  return err;
+}
+
+// Cylind::dreamcylind without attenuation (src/dreamcylind.cc:78-273).
+int dro_dreamcylind(const double *Ro, idx_t No, double a, double b, double Rc, double dx,
+                    double dy, double dt, idx_t nt, int delay_type, const double *delay,
+                    double v, double cp, double *h, int level, int nthreads,
+                    long long *oob_total)
+{
+  std::atomic<bool> running(true);
+  std::atomic<long long> oob_sum(0);
+  int err = run_partitioned(No, nthreads, [&](idx_t start, idx_t stop, int &status) {
+    long long oob = 0;
+    for (idx_t n = start; n < stop; n++) {
+      const double dl = (delay_type == D_SINGLE) ? delay[0] : delay[n];
+      int e = cylind_sum(Ro[n], Ro[n + No], Ro[n + 2 * No], a, b, Rc, dx, dy, dt, nt, dl, v, cp,
+                         &h[n * nt], level, 1.0, &oob);
+      if (e != E_NONE) status = e;
+      if (e == E_OOB) running = false;
+      if (!running) break;
+    }
+    oob_sum += oob;
+  });
+  if (oob_total) *oob_total = oob_sum.load();
+  return err;
+}
+
+// ArrCylind::dream_arr_cylind without attenuation (src/dream_arr_cylind.cc:
+// 90-195, 197-250, 313-437): the element loop of dream_arr_rect with the
+// curved-element sum.
+int dro_dream_arr_cylind(const double *Ro, idx_t No, double a, double b, double Rc, double dx,
+                         double dy, double dt, idx_t nt, int delay_type, const double *delay,
+                         double v, double cp, idx_t L, const double *G, int foc_met,
+                         const double *focal, int steer_met, double theta, double phi,
+                         const double *apod, int do_apod, int apod_met, double apod_par,
+                         double *h, int level, int nthreads, long long *oob_total)
+{
+  const double *gx = G, *gy = G + L, *gz = G + 2 * L;
+  std::atomic<bool> running(true);
+  std::atomic<long long> oob_sum(0);
+  int err = run_partitioned(No, nthreads, [&](idx_t start, idx_t stop, int &status) {
+    long long oob = 0;
+    for (idx_t n = start; n < stop; n++) {
+      const double xo = Ro[n], yo = Ro[n + No], zo = Ro[n + 2 * No];
+      const double dl = (delay_type == D_SINGLE) ? delay[0] : delay[n];
+      double x_max, y_max, r_max;
+      dro_max_dim_arr(gx, gy, L, &x_max, &y_max, &r_max);
+      int e = E_NONE;
+      for (idx_t el = 0; el < L; el++) {
+        double foc = (foc_met == F_UD) ? focal[el]
+                   : dro_focusing(foc_met, focal[0], gx[el], gy[el], x_max, y_max, r_max, cp);
+        double steer = dro_beamsteering(steer_met, theta, phi, gx[el], gy[el], x_max, y_max, cp);
+        double w = do_apod ? dro_apodization(apod_met, el, apod, 1.0, gx[el], gy[el], r_max, apod_par)
+                           : 1.0;
+        e = cylind_sum(xo - gx[el], yo - gy[el], zo - gz[el], a, b, Rc, dx, dy, dt, nt,
+                       dl - foc - steer, v, cp, &h[n * nt], level, w, &oob);
+        if (e == E_OOB) break;
+      }
+      if (e != E_NONE) status = e;
+      if (e == E_OOB) running = false;
+      if (!running) break;
+    }
+    oob_sum += oob;
+  });
+  if (oob_total) *oob_total = oob_sum.load();
+  return err;
+}
+
+// ---------------------------------------------------------------------------
+// Annular array, src/dream_arr_annu.cc.  G holds radii: Gr[0] is the centre
+// disc, then an (inner, outer) pair per ring; ring n is the outer disc 2n
+// minus the inner disc 2n-1, and there are (num_radii+1)/2 rings.
+// ---------------------------------------------------------------------------
+
+// Ring radius used by focusing and apodization (:361-368): 0 for the centre
+// disc, the mean of the pair's radii for the others.
+void dro_annu_ring_radii(const double *Gr, idx_t num_radii, double *ring_r)
+{
+  const idx_t rings = (num_radii + 1) / 2;
+  if (rings > 0) ring_r[0] = 0.0;
+  for (idx_t n = 1; n < rings && 2 * n < num_radii; n++) ring_r[n] = (Gr[2 * n] + Gr[2 * n - 1]) / 2.0;
+}
+
+// Focusing delay in us (:376-405): the four geometric methods share one
+// spherical law about the ring radius; 'ud' returns the given delay.
+double dro_annu_focusing(int foc_met, double focal, double ring_r, double ring_r_max, double cp)
+{
+  if (foc_met == F_X || foc_met == F_Y || foc_met == F_XY || foc_met == F_X_Y) {
+    const double far = std::sqrt(ring_r_max * ring_r_max + focal * focal);
+    const double near = std::sqrt(ring_r * ring_r + focal * focal);
+    return (far - near) * 1000 / cp;
+  }
+  if (foc_met == F_UD) return focal;
+  return 0.0;
+}
+
+// Apodization weight of ring n (:413-447); Hann and Hamming have no annular
+// form and give 1.
+double dro_annu_apodization(int apod_met, idx_t n, const double *apod, double ring_r,
+                            double ring_r_max, double apod_par)
+{
+  const double q = ring_r * ring_r / (ring_r_max * ring_r_max);
+  switch (apod_met) {
+  case A_UD: return apod[n];
+  case A_TRIANGLE: return 1.0 - std::fabs(ring_r) / ring_r_max;
+  case A_GAUSS: return std::exp(-(apod_par * ring_r * ring_r) / (ring_r_max * ring_r_max));
+  case A_RAISED: return apod_par + std::cos(ring_r * M_PI / ring_r_max);
+  case A_SIMPLY: return 1.0 - q;
+  case A_CLAMPED: return (1.0 - q) * (1.0 - q);
+  default: return 1.0;
+  }
+}
+
+// ArrAnnu::dream_arr_annu without attenuation (:186-254 per point, :514-634
+// threads).  Per point: every disc response through the disc sum (disc 0 is
+// skipped, left zero, when Gr[0] <= DBL_EPSILON); the point's status is the
+// last disc's (an E_OOB stops at once).  Each ring response is then moved
+// later by rint(foc/dt) + 1 samples, cut at nt, and added with its weight
+// (:459-481).  A shift that is not a number adds nothing (the reference's
+// integer conversion of it is undefined).
+int dro_dream_arr_annu(const double *Ro, idx_t No, double dx, double dy, double dt, idx_t nt,
+                       int delay_type, const double *delay, double v, double cp,
+                       idx_t num_radii, const double *Gr, int foc_met, const double *focal,
+                       const double *apod, int do_apod, int apod_met, double apod_par,
+                       double *h, int level, int nthreads, long long *oob_total)
+{
+  const idx_t rings = (num_radii + 1) / 2;
+  if (rings < 1) {
+    std::fill(h, h + No * nt, 0.0);
+    return E_NONE;
+  }
+  std::vector<double> ring_r((size_t) rings), weight((size_t) rings), shift((size_t) rings);
+  dro_annu_ring_radii(Gr, num_radii, ring_r.data());
+  const double r_max = ring_r[(size_t) rings - 1];
+  for (idx_t n = 0; n < rings; n++) {
+    const double f = dro_annu_focusing(foc_met, foc_met == F_UD ? focal[n] : focal[0],
+                                       ring_r[(size_t) n], r_max, cp);
+    weight[(size_t) n] = do_apod ? dro_annu_apodization(apod_met, n, apod, ring_r[(size_t) n],
+                                                        r_max, apod_par)
+                                 : 1.0;
+    shift[(size_t) n] = std::rint(f / dt) + 1.0;
+  }
+  std::atomic<bool> running(true);
+  std::atomic<long long> oob_sum(0);
+  int err = run_partitioned(No, nthreads, [&](idx_t start, idx_t stop, int &status) {
+    long long oob = 0;
+    std::vector<double> disc((size_t) (num_radii * nt));
+    for (idx_t p = start; p < stop; p++) {
+      const double dl = (delay_type == D_SINGLE) ? delay[0] : delay[p];
+      std::fill(disc.begin(), disc.end(), 0.0);
+      int e = E_NONE;
+      for (idx_t k = 0; k < num_radii; k++) {
+        if (k == 0 && Gr[0] <= 2.220446049250313e-16) continue;
+        e = circ_sum(Ro[p], Ro[p + No], Ro[p + 2 * No], Gr[k], dx, dy, dt, nt, dl, v, cp,
+                     &disc[(size_t) (k * nt)], level, 1.0, &oob);
+        if (e == E_OOB) break;
+      }
+      double *hp = &h[p * nt];
+      for (idx_t i = 0; i < nt; i++) hp[i] = 0.0;
+      if (e != E_OOB) {
+        for (idx_t n = 0; n < rings; n++) {
+          const double *outer = &disc[(size_t) (2 * n * nt)];
+          const double *inner = n ? &disc[(size_t) ((2 * n - 1) * nt)] : nullptr;
+          const double s = shift[(size_t) n];
+          for (idx_t i = 0; i < nt; i++) {
+            const double j = (double) i - s;   // source sample of output sample i
+            double src = 0.0;
+            if (s < (double) nt && j >= 0.0 && j < (double) nt) {
+              const idx_t jj = (idx_t) j;
+              src = inner ? outer[jj] - inner[jj] : outer[jj];
+            }
+            hp[i] += weight[(size_t) n] * src;
+          }
+        }
       }
       if (e != E_NONE) status = e;
       if (e == E_OOB) running = false;

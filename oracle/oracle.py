@@ -87,6 +87,26 @@ def lib():
             ctypes.c_int, _c_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _c_dp,
             ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_dp, ctypes.c_int, ctypes.c_int,
             _c_lp]
+        L.dro_dreamcylind.restype = ctypes.c_int
+        L.dro_dreamcylind.argtypes = [_c_dp, _c_i64] + [ctypes.c_double] * 6 + [
+            _c_i64, ctypes.c_int, _c_dp, ctypes.c_double, ctypes.c_double, _c_dp,
+            ctypes.c_int, ctypes.c_int, _c_lp]
+        L.dro_dream_arr_cylind.restype = ctypes.c_int
+        L.dro_dream_arr_cylind.argtypes = [_c_dp, _c_i64] + [ctypes.c_double] * 6 + [
+            _c_i64, ctypes.c_int, _c_dp, ctypes.c_double, ctypes.c_double, _c_i64, _c_dp,
+            ctypes.c_int, _c_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _c_dp,
+            ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_dp, ctypes.c_int, ctypes.c_int,
+            _c_lp]
+        L.dro_dream_arr_annu.restype = ctypes.c_int
+        L.dro_dream_arr_annu.argtypes = [_c_dp, _c_i64] + [ctypes.c_double] * 3 + [
+            _c_i64, ctypes.c_int, _c_dp, ctypes.c_double, ctypes.c_double, _c_i64, _c_dp,
+            ctypes.c_int, _c_dp, _c_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_dp,
+            ctypes.c_int, ctypes.c_int, _c_lp]
+        L.dro_annu_ring_radii.argtypes = [_c_dp, _c_i64, _c_dp]
+        L.dro_annu_focusing.restype = ctypes.c_double
+        L.dro_annu_focusing.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4
+        L.dro_annu_apodization.restype = ctypes.c_double
+        L.dro_annu_apodization.argtypes = [ctypes.c_int, _c_i64, _c_dp] + [ctypes.c_double] * 3
         for name, ptr, scal in (("dro_das_f64", _c_dp, ctypes.c_double),
                                 ("dro_das_f32", _c_fp, ctypes.c_float)):
             f = getattr(L, name)
@@ -248,6 +268,81 @@ def dream_arr_circ(Ro, R, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=
         APOD.get(apod_met, 0), apod_par, _p(H), ERR_LEVEL[err_level], int(nthreads),
         ctypes.byref(oob))
     return H, err, oob.value
+
+
+def dreamcylind(Ro, a, b, Rcurv, dx, dy, dt, nt, delay, v, cp, err_level="stop", nthreads=0):
+    """Cylind::dreamcylind (no attenuation).  Returns (H, err code, oob count)."""
+    Ro = _f64(np.atleast_2d(Ro))
+    No = Ro.shape[0]
+    delay = _f64(np.atleast_1d(np.asarray(delay, dtype=np.float64).ravel()))
+    H = np.zeros((int(nt), No), dtype=np.float64, order="F")
+    oob = ctypes.c_longlong(0)
+    err = lib().dro_dreamcylind(_p(Ro), No, a, b, Rcurv, dx, dy, dt, int(nt),
+                                0 if delay.size == 1 else 1, _p(delay), v, cp, _p(H),
+                                ERR_LEVEL[err_level], int(nthreads), ctypes.byref(oob))
+    return H, err, oob.value
+
+
+def dream_arr_cylind(Ro, a, b, Rcurv, dx, dy, dt, nt, delay, v, cp, G, foc_met="off",
+                     focal=0.0, steer_met="off", theta=0.0, phi=0.0, apod_met="off", apod=None,
+                     apod_par=0.0, err_level="stop", nthreads=0):
+    """ArrCylind::dream_arr_cylind (no attenuation).  Returns (H, err code, oob count)."""
+    Ro = _f64(np.atleast_2d(Ro))
+    G = _f64(np.atleast_2d(G))
+    No, L = Ro.shape[0], G.shape[0]
+    delay = _f64(np.atleast_1d(np.asarray(delay, dtype=np.float64).ravel()))
+    focal = _f64(np.atleast_1d(np.asarray(focal, dtype=np.float64).ravel()))
+    if focal.size < L:
+        focal = _f64(np.resize(focal, L))
+    do_apod = apod_met != "off"
+    apod_v = _f64(np.ones(L) if apod is None else np.asarray(apod, dtype=np.float64).ravel())
+    H = np.zeros((int(nt), No), dtype=np.float64, order="F")
+    oob = ctypes.c_longlong(0)
+    err = lib().dro_dream_arr_cylind(
+        _p(Ro), No, a, b, Rcurv, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1, _p(delay), v,
+        cp, L, _p(G), FOC[foc_met], _p(focal), STEER[steer_met], theta, phi, _p(apod_v),
+        int(do_apod), APOD.get(apod_met, 0), apod_par, _p(H), ERR_LEVEL[err_level],
+        int(nthreads), ctypes.byref(oob))
+    return H, err, oob.value
+
+
+def dream_arr_annu(Ro, Gr, dx, dy, dt, nt, delay, v, cp, foc_met="off", focal=0.0,
+                   apod_met="off", apod=None, apod_par=0.0, err_level="stop", nthreads=0):
+    """ArrAnnu::dream_arr_annu (no attenuation).  Gr is the vector of radii (centre
+    disc, then inner/outer per ring).  Returns (H, err code, oob count)."""
+    Ro = _f64(np.atleast_2d(Ro))
+    Gr = _f64(np.asarray(Gr, dtype=np.float64).ravel())
+    No, nr = Ro.shape[0], Gr.size
+    delay = _f64(np.atleast_1d(np.asarray(delay, dtype=np.float64).ravel()))
+    focal = _f64(np.atleast_1d(np.asarray(focal, dtype=np.float64).ravel()))
+    if focal.size < nr:
+        focal = _f64(np.resize(focal, max(nr, 1)))
+    do_apod = apod_met != "off"
+    apod_v = _f64(np.ones(max(nr, 1)) if apod is None
+                  else np.asarray(apod, dtype=np.float64).ravel())
+    H = np.zeros((int(nt), No), dtype=np.float64, order="F")
+    oob = ctypes.c_longlong(0)
+    err = lib().dro_dream_arr_annu(
+        _p(Ro), No, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1, _p(delay), v, cp, nr,
+        _p(Gr), FOC[foc_met], _p(focal), _p(apod_v), int(do_apod), APOD.get(apod_met, 0),
+        apod_par, _p(H), ERR_LEVEL[err_level], int(nthreads), ctypes.byref(oob))
+    return H, err, oob.value
+
+
+def annu_ring_radii(Gr):
+    Gr = _f64(np.asarray(Gr, dtype=np.float64).ravel())
+    out = np.zeros((Gr.size + 1) // 2)
+    lib().dro_annu_ring_radii(_p(Gr), Gr.size, _p(out))
+    return out
+
+
+def annu_focusing(foc_met, focal, ring_r, ring_r_max, cp):
+    return lib().dro_annu_focusing(FOC[foc_met], focal, ring_r, ring_r_max, cp)
+
+
+def annu_apodization(apod_met, n, apod, ring_r, ring_r_max, apod_par=0.0):
+    apod = _f64(np.atleast_1d(np.asarray(apod if apod is not None else [1.0], dtype=np.float64)))
+    return lib().dro_annu_apodization(APOD[apod_met], n, _p(apod), ring_r, ring_r_max, apod_par)
 
 
 # ----------------------------------------------------------------------------- DAS

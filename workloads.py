@@ -128,6 +128,20 @@ def c3(seed=3, n_problems=32):
     return probs
 
 
+def annu():
+    """Annular array of the reference's own test (radii 3.5 .. 8 mm, four
+    rings around a centre disc, x-focused at 25 mm; 4 MHz sampling, nt 300,
+    dx = dy = 0.05 mm) over a 64 x 64 grid of points, x in [-10, 10] mm,
+    z in [10, 50] mm."""
+    gr = [3.5, 3.6, 5.0, 5.1, 6.0, 6.1, 7.0, 7.1, 8.0]
+    x = np.linspace(-10.0, 10.0, 64)
+    z = np.linspace(10.0, 50.0, 64)
+    X, Z = np.meshgrid(x, z)
+    Ro = np.c_[X.ravel("F"), np.zeros(X.size), Z.ravel("F")]
+    return dict(Ro=Ro, Gr=gr, dx=0.05, dy=0.05, dt=0.25, nt=300, delay=0.0, v=1.0, cp=1000.0,
+                foc_met="x", focal=25.0, err_level="ignore")
+
+
 def sir_small_uneven_batch(seed=13, n_problems=128):
     """128 small dream_arr_rect problems of uneven size: 16-256 points, 4-16
     elements of 0.2-1.0 x 2-6 mm, nt 200-800, the rest as in c3()."""
