@@ -178,7 +178,36 @@ def bench_sir(steps, warmup):
         roofline_frac=(inb * 8 + out_bytes) / (ms_b * 1e-3) / HBM_PEAK,
         per_call_ms=round(ms_p, 3), speedup_vs_per_call=ms_p / ms_b)
     out.update(bench_annu(steps, warmup))
+    out.update(bench_att(steps, warmup))
     return out
+
+
+def bench_att(steps, warmup):
+    """C1 with attenuation (alpha = 5 dB/(cm MHz)): 64 points x 16 elements x
+    3600 sub-elements, nt = 400, FP64, through the host entry point (copies
+    included).  Each in-range contribution costs nt/2 complex exponentials."""
+    import time
+
+    import numpy as np
+
+    import workloads
+    from dream_amd.dream_arr_rect import dream_arr_rect
+    w = workloads.c1()
+
+    def run():
+        return dream_arr_rect(w["Ro"], [w["a"], w["b"]], w["G"], [w["dx"], w["dy"], w["dt"], w["nt"]],
+                              w["delay"], [w["v"], w["cp"], 5.0], "off", 0.0, "off", [0.0, 0.0],
+                              "off", None, 0.0, "ignore")
+    for _ in range(max(warmup, 1)):
+        run()
+    n = max(2, steps // 10)
+    t = time.perf_counter()
+    for _ in range(n):
+        run()
+    ms = (time.perf_counter() - t) * 1e3 / n
+    contrib = _sir_contributions(w)
+    return {"c1_sir_attenuated_f64": dict(ms=round(ms, 3), contributions=int(contrib),
+                                          exponentials_per_s=contrib * (w["nt"] // 2) / (ms * 1e-3))}
 
 
 def _disc_subelements(R, d):

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdream_hip.so")
-SOURCES = ["sir.hip", "annu.hip", "das.hip", "conv.hip", "tune.hip", "abi.hip"]
+SOURCES = ["sir.hip", "annu.hip", "att.hip", "das.hip", "conv.hip", "tune.hip", "abi.hip"]
 
 HIPCC = os.environ.get("HIPCC") or shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLAGS = [

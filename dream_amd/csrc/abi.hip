@@ -181,8 +181,6 @@ void run_single(const SirProblem &p, int out_dtype, hipStream_t s, int *sir_err,
 int run_host_sir(SirProblem p, double alpha, const double *Ro, const double *delay, double *h,
                  int *sir_err)
 {
-  if (alpha > 2.220446049250313e-16)
-    throw Error(DREAM_HIP_EUNSUPPORTED, "SIR: attenuation (alpha > 0) is not on the GPU path");
   require(p.No >= 0 && p.nt >= 1, "SIR: need No >= 0 and nt >= 1");
   require(p.No == 0 || (Ro && delay && h), "SIR: null argument");
   hipStream_t s = lib_stream();
@@ -193,7 +191,13 @@ int run_host_sir(SirProblem p, double alpha, const double *Ro, const double *del
   p.delay = (const double *) dD.p;
   p.h = dH.p;
   int code = 0;
-  run_single(p, DREAM_HIP_F64, s, &code, nullptr);
+  if (alpha > 2.220446049250313e-16) {
+    int32_t c = 0;
+    run_sir_att(std::vector<SirProblem>{p}, alpha, DREAM_HIP_F64, s, &c, nullptr);
+    code = c;
+  } else {
+    run_single(p, DREAM_HIP_F64, s, &code, nullptr);
+  }
   if (p.No) DREAM_HIP_CHECK(hipMemcpy(h, dH.p, (size_t) (p.No * p.nt) * 8, hipMemcpyDeviceToHost));
   if (sir_err) *sir_err = code;
   return DREAM_HIP_OK;
@@ -257,23 +261,11 @@ int dream_hip_arr_rect(double alpha, const double *Ro, int64_t No, double a, dou
                        double apod_par, double *h, int err_level, int *sir_err)
 {
   return guarded([&]() {
-    if (alpha > 2.220446049250313e-16)
-      throw Error(DREAM_HIP_EUNSUPPORTED, "dream_arr_rect: attenuation (alpha > 0) is not on the GPU path");
-    require(No >= 0 && nt >= 1, "dream_arr_rect: need No >= 0 and nt >= 1");
     require(No == 0 || (Ro && delay && h), "dream_arr_rect: null argument");
-    hipStream_t s = lib_stream();
-    const int64_t nd = delay_type ? No : 1;
-    DevBuf dRo(Ro, (size_t) No * 3 * 8, s), dD(delay, (size_t) nd * 8, s);
-    DevBuf dH((size_t) No * nt * 8);
-    int code = 0;
-    int rc = dream_hip_arr_rect_dev((const double *) dRo.p, No, a, b, dx, dy, dt, nt, delay_type,
-                                    (const double *) dD.p, v, cp, num_elements, G, foc_met, focal,
-                                    steer_met, theta, phi, apod, do_apod, apod_met, apod_par, dH.p,
-                                    DREAM_HIP_F64, err_level, &code, nullptr, s);
-    if (rc != DREAM_HIP_OK) throw Error(rc, g_last_error);
-    if (No) DREAM_HIP_CHECK(hipMemcpy(h, dH.p, (size_t) No * nt * 8, hipMemcpyDeviceToHost));
-    if (sir_err) *sir_err = code;
-    return DREAM_HIP_OK;
+    return run_host_sir(make_problem(nullptr, No, a, b, dx, dy, dt, nt, delay_type, nullptr, v,
+                                     cp, num_elements, G, foc_met, focal, steer_met, theta, phi,
+                                     apod, do_apod, apod_met, apod_par, nullptr, err_level),
+                        alpha, Ro, delay, h, sir_err);
   });
 }
 
@@ -408,21 +400,26 @@ int dream_hip_arr_annu(double alpha, const double *Ro, int64_t No, double dx, do
                        double apod_par, double *h, int err_level, int *sir_err)
 {
   return guarded([&]() {
-    if (alpha > 2.220446049250313e-16)
-      throw Error(DREAM_HIP_EUNSUPPORTED,
-                  "dream_arr_annu: attenuation (alpha > 0) is not on the GPU path");
     require(No >= 0 && nt >= 1, "dream_arr_annu: need No >= 0 and nt >= 1");
     require(No == 0 || (Ro && delay && h), "dream_arr_annu: null argument");
     hipStream_t s = lib_stream();
     const int64_t nd = delay_type ? No : 1;
     DevBuf dRo(Ro, (size_t) No * 3 * 8, s), dD(delay, (size_t) nd * 8, s);
     DevBuf dH((size_t) No * nt * 8);
+    require(err_level >= 0 && err_level <= 4, "dream_arr_annu: unknown error level");
+    AnnuProblem an;
+    an.Ro = (const double *) dRo.p;  an.No = No;
+    an.dx = dx;  an.dy = dy;  an.dt = dt;  an.nt = nt;
+    an.delay_type = delay_type;  an.delay = (const double *) dD.p;
+    an.v = v;  an.cp = cp;
+    an.num_radii = num_radii;  an.Gr = Gr;
+    an.foc_met = foc_met;  an.focal = focal;
+    an.apod = apod;  an.do_apod = do_apod;  an.apod_met = apod_met;  an.apod_par = apod_par;
+    an.err_level = err_level;
+    an.alpha = alpha;
+    an.h = dH.p;
     int code = 0;
-    int rc = dream_hip_arr_annu_dev((const double *) dRo.p, No, dx, dy, dt, nt, delay_type,
-                                    (const double *) dD.p, v, cp, num_radii, Gr, foc_met, focal,
-                                    apod, do_apod, apod_met, apod_par, dH.p, DREAM_HIP_F64,
-                                    err_level, &code, nullptr, s);
-    if (rc != DREAM_HIP_OK) throw Error(rc, g_last_error);
+    run_arr_annu(an, DREAM_HIP_F64, s, &code, nullptr);
     if (No) DREAM_HIP_CHECK(hipMemcpy(h, dH.p, (size_t) No * nt * 8, hipMemcpyDeviceToHost));
     if (sir_err) *sir_err = code;
     return DREAM_HIP_OK;

@@ -165,11 +165,15 @@ void run_arr_annu(const AnnuProblem &a, int out_dtype, hipStream_t s, int *sir_e
     int code = E_NONE;
     int64_t oob = 0;
     if (!probs.empty()) {
-      SirPlan plan(probs, DREAM_HIP_F64);
-      plan.run(s, false);
       std::vector<int32_t> codes(probs.size());
       std::vector<int64_t> oobs(probs.size());
-      plan.status(codes.data(), oobs.data());   // synchronises `s`
+      if (a.alpha > 2.220446049250313e-16) {
+        run_sir_att(probs, a.alpha, DREAM_HIP_F64, s, codes.data(), oobs.data());
+      } else {
+        SirPlan plan(probs, DREAM_HIP_F64);
+        plan.run(s, false);
+        plan.status(codes.data(), oobs.data());   // synchronises `s`
+      }
       // A point's status is its last disc's; a stop ends the call at once.
       for (size_t q = 0; q < probs.size(); q++) {
         oob += oobs[q];

@@ -114,6 +114,13 @@ private:
   bool ran_ = false;
 };
 
+// SIR with attenuation (alpha > DBL_EPSILON, in dB/(cm MHz)) of a list of
+// problems, each written to its own h (overwritten); per-problem SIRError
+// codes and out-of-bounds counts as SirPlan::status reports them.
+// Synchronises `s`.
+void run_sir_att(const std::vector<SirProblem> &problems, double alpha, int out_dtype,
+                 hipStream_t s, int32_t *codes, int64_t *oobs);
+
 // One dream_arr_annu call (src/dream_arr_annu.cc:514-634): Gr, focal and
 // apod are host arrays, Ro, delay and h device pointers.
 struct AnnuProblem {
@@ -125,6 +132,7 @@ struct AnnuProblem {
   int foc_met = 0;  const double *focal = nullptr;
   const double *apod = nullptr;  int do_apod = 0, apod_met = 0;  double apod_par = 0;
   int err_level = L_STOP;
+  double alpha = 0;    // attenuation [dB/(cm MHz)]; > DBL_EPSILON selects it
   void *h = nullptr;   // nt x No, FP64 or FP32
 };
 

@@ -9,8 +9,8 @@ then an (inner, outer) pair per ring; as in the reference, the radii are the
 first ``G.shape[0]`` entries (a 1-D array counts as a column).  The focusing
 delays ('ud') and apodization weights ('ud') are vectors of the same length
 as G, of which one value per ring is used.  ``win_par`` is accepted and not
-used (include/arg_parser_python.h:536-540).  Attenuation (alpha > 0) is not
-on this path and raises.
+used (include/arg_parser_python.h:536-540).  Attenuation (alpha > 0) runs
+the attenuated sum of src/attenuation.cc:72-140.
 """
 import ctypes
 

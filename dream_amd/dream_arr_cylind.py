@@ -9,8 +9,8 @@ checks and messages, same output (an nt x No float64 array in Fortran
 order), same error behaviour (``"stop"`` with an out-of-bounds sample prints
 ``SIR out-of-bounds!`` and raises ``RuntimeError``; ``"warn"`` prints a
 warning).  As in the reference's Python gateway, ``win_par`` is accepted and
-not used (include/arg_parser_python.h:536-540).  Attenuation (alpha > 0) is
-not on this path and raises.  With b > 2|Rcurv| the reference prints an
+not used (include/arg_parser_python.h:536-540).  Attenuation (alpha > 0)
+runs the attenuated sum of src/attenuation.cc:72-140.  With b > 2|Rcurv| the reference prints an
 error message per element evaluation and yields a zero response; this module
 prints it once and yields the same zeros.
 """

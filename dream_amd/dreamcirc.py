@@ -3,7 +3,8 @@
 Drop-in for the reference's Python module ``dreamcirc``
 (python/src/py_dreamcirc.cc:34-157, 226-237): the SIR of one circular
 element centred at the origin; same checks, output (nt x No float64, Fortran
-order) and error behaviour.  Attenuation (alpha > 0) is not on this path.
+order) and error behaviour.  Attenuation (alpha > 0) runs the
+attenuated sum of src/attenuation.cc:72-140.
 """
 import ctypes
 

@@ -7,7 +7,8 @@ focused/concave, < 0 defocused/convex); same checks, output (nt x No
 float64, Fortran order) and error behaviour.  With b > 2|Rcurv| the
 reference prints an error message per observation point and yields a zero
 response; this module prints it once and yields the same zeros.
-Attenuation (alpha > 0) is not on this path.
+Attenuation (alpha > 0) runs the
+attenuated sum of src/attenuation.cc:72-140.
 """
 import ctypes
 

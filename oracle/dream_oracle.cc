@@ -134,7 +134,83 @@ static int run_partitioned(idx_t No, int forced_threads, Body body)
   return err;
 }
 
+// ---------------------------------------------------------------------------
+// Attenuation, Attenuation::att (src/attenuation.cc:72-140).  With alpha >
+// DBL_EPSILON [dB/(cm MHz)] every in-range contribution adds ai times the
+// lossy Green's function of its distance r [mm] and sample it instead of ai
+// at it alone: the spectrum X[0] = 1, for k = 1 .. n/2
+//   X[k] = exp(-r a w/(2 pi)) * exp(i(-w r a b/pi^2 - t w)),
+//   w = k (2 pi/n) Fs, b = log(1/(a1 w)), a1 = dt 0.95/pi, t = it dt,
+//   a = alpha/(8.686 * 10^4) [Np/(m Hz)], r and dt in SI units,
+// and X[n-k] = conj(X[k]) (which for even n leaves X[n/2] conjugated), then
+// FFTW's complex-to-real inverse transform of length n (only X[0 .. n/2] is
+// read; the imaginary parts of X[0] and X[n/2] drop out) divided by n.  The
+// inverse transform is evaluated here as the direct sum.
+// ---------------------------------------------------------------------------
+
+static double g_alpha = 0.0;
+
+static inline bool attenuated() { return g_alpha > 2.220446049250313e-16; }
+
+static void att_add(double r, idx_t it, double *h, double ai, idx_t n, double dt_us)
+{
+  thread_local std::vector<double> re, im, cs, sn, x;
+  thread_local idx_t tw_n = -1;
+  const idx_t half = n / 2;
+  re.assign((size_t) half + 1, 0.0);
+  im.assign((size_t) half + 1, 0.0);
+  x.assign((size_t) n, 0.0);
+  if (tw_n != n) {
+    cs.resize((size_t) n);
+    sn.resize((size_t) n);
+    for (idx_t m = 0; m < n; m++) {
+      cs[(size_t) m] = std::cos(2.0 * M_PI * (double) m / (double) n);
+      sn[(size_t) m] = std::sin(2.0 * M_PI * (double) m / (double) n);
+    }
+    tw_n = n;
+  }
+  const double alpha = g_alpha / (8.686 * 10000.0);
+  const double dt = dt_us * 1.0e-6;
+  const double Fs = 1 / dt;
+  const double dw = 2.0 * M_PI / double(n);
+  const double t = double(it) * dt;
+  r *= 1.0e-3;
+  const double a1 = dt * 0.95 / M_PI;
+  re[0] = 1.0;
+  for (idx_t k = 1; k < n / 2 + 1; k++) {
+    const double w = (double(k) * dw) * Fs;
+    const double x1 = std::exp(-r * alpha * w / (2.0 * M_PI));
+    const double b = std::log(1.0 / (a1 * w));
+    const double arg = -w * r * alpha / (M_PI * M_PI) * b - t * w;
+    re[(size_t) k] = x1 * std::cos(arg);
+    im[(size_t) k] = (2 * k == n) ? -(x1 * std::sin(arg)) : x1 * std::sin(arg);
+  }
+  // Complex-to-real inverse DFT: x[j] = X0 + 2 sum_k Re(X[k] e^{+2 pi i jk/n})
+  // (+ X[n/2] (-1)^j for even n), then / n.
+  const idx_t kmax = (n % 2 == 0) ? half - 1 : half;
+  for (idx_t j = 0; j < n; j++) {
+    double acc = re[0];
+    for (idx_t k = 1; k <= kmax; k++) {
+      const idx_t m = (j * k) % n;
+      acc += 2.0 * (re[(size_t) k] * cs[(size_t) m] - im[(size_t) k] * sn[(size_t) m]);
+    }
+    if (n % 2 == 0) acc += (j % 2 == 0) ? re[(size_t) half] : -re[(size_t) half];
+    x[(size_t) j] = acc / double(n);
+  }
+  for (idx_t j = 0; j < n; j++) h[j] += ai * x[(size_t) j];
+}
+
+// Adds one in-range contribution: ai at sample it, or its attenuated pulse.
+static inline void deposit(double *h, idx_t it, double ai, double r, idx_t nt, double dt)
+{
+  if (attenuated()) att_add(r, it, h, ai, nt, dt); else h[it] += ai;
+}
+
 extern "C" {
+
+// Attenuation coefficient [dB/(cm MHz)] used by the SIR sums that follow
+// (0: none).  Set it before a call; it is read by every thread of the call.
+void dro_set_alpha(double alpha) { g_alpha = alpha; }
 
 // ---------------------------------------------------------------------------
 // Array helpers, src/arr_functions.cc:60-262.
@@ -296,7 +372,7 @@ static int rect_sum(double xo, double yo, double zo, double a, double b,
       const double u = std::rint((t - delay) / dt);
       const bool inside = (u >= 0.0) && (u < (double) nt);
       if (inside) {
-        h[(idx_t) u] += ai;
+        deposit(h, (idx_t) u, ai, r, nt, dt);
       } else {
         if (oob) (*oob)++;
         err = oob_code(level);
@@ -436,7 +512,7 @@ static int circ_sum(double xo, double yo, double zo, double R, double dx, double
       const double u = std::rint((t - delay) / dt);
       const bool inside = (u >= 0.0) && (u < (double) nt);
       if (inside) {
-        h[(idx_t) u] += ai;
+        deposit(h, (idx_t) u, ai, r, nt, dt);
       } else {
         if (oob) (*oob)++;
         err = oob_code(level);
@@ -549,7 +625,7 @@ static int cylind_sum(double xo, double yo, double zo, double a, double b, doubl
       const double t = r * 1.0e3 / cp;
       const double u = std::rint((t - delay) / dt);
       if (u >= 0.0 && u < (double) nt) {
-        h[(idx_t) u] += ai;
+        deposit(h, (idx_t) u, ai, r, nt, dt);
       } else {
         if (oob) (*oob)++;
         err = oob_code(level);

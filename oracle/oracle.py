@@ -97,6 +97,8 @@ def lib():
             ctypes.c_int, _c_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _c_dp,
             ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_dp, ctypes.c_int, ctypes.c_int,
             _c_lp]
+        L.dro_set_alpha.argtypes = [ctypes.c_double]
+        L.dro_set_alpha.restype = None
         L.dro_dream_arr_annu.restype = ctypes.c_int
         L.dro_dream_arr_annu.argtypes = [_c_dp, _c_i64] + [ctypes.c_double] * 3 + [
             _c_i64, ctypes.c_int, _c_dp, ctypes.c_double, ctypes.c_double, _c_i64, _c_dp,
@@ -197,7 +199,21 @@ def rect_contributions(xo, yo, zo, a, b, dx, dy, dt, delay, v, cp, weight=1.0):
 
 # ----------------------------------------------------------------------------- SIR
 
-def dreamrect(Ro, a, b, dx, dy, dt, nt, delay, v, cp, err_level="stop", nthreads=0):
+
+class _alpha:
+    """Sets the oracle's attenuation coefficient [dB/(cm MHz)] for one call."""
+
+    def __init__(self, alpha):
+        self.alpha = float(alpha)
+
+    def __enter__(self):
+        lib().dro_set_alpha(self.alpha)
+
+    def __exit__(self, *exc):
+        lib().dro_set_alpha(0.0)
+
+
+def dreamrect(Ro, a, b, dx, dy, dt, nt, delay, v, cp, err_level="stop", nthreads=0, alpha=0.0):
     """Rect::dreamrect (no attenuation).  Returns (H nt x No, err code, oob count)."""
     Ro = _f64(np.atleast_2d(Ro))
     No = Ro.shape[0]
@@ -205,14 +221,15 @@ def dreamrect(Ro, a, b, dx, dy, dt, nt, delay, v, cp, err_level="stop", nthreads
     dtype = 0 if delay.size == 1 else 1
     H = np.zeros((int(nt), No), dtype=np.float64, order="F")
     oob = ctypes.c_longlong(0)
-    err = lib().dro_dreamrect(_p(Ro), No, a, b, dx, dy, dt, int(nt), dtype, _p(delay), v, cp,
-                              _p(H), ERR_LEVEL[err_level], int(nthreads), ctypes.byref(oob))
+    with _alpha(alpha):
+        err = lib().dro_dreamrect(_p(Ro), No, a, b, dx, dy, dt, int(nt), dtype, _p(delay), v, cp,
+                                  _p(H), ERR_LEVEL[err_level], int(nthreads), ctypes.byref(oob))
     return H, err, oob.value
 
 
 def dream_arr_rect(Ro, a, b, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=0.0,
                    steer_met="off", theta=0.0, phi=0.0, apod_met="off", apod=None,
-                   apod_par=0.0, err_level="stop", nthreads=0):
+                   apod_par=0.0, err_level="stop", nthreads=0, alpha=0.0):
     """ArrRect::dream_arr_rect (no attenuation).  Returns (H, err code, oob count)."""
     Ro = _f64(np.atleast_2d(Ro))
     G = _f64(np.atleast_2d(G))
@@ -226,30 +243,32 @@ def dream_arr_rect(Ro, a, b, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", foc
     apod_v = _f64(np.ones(L) if apod is None else np.asarray(apod, dtype=np.float64).ravel())
     H = np.zeros((int(nt), No), dtype=np.float64, order="F")
     oob = ctypes.c_longlong(0)
-    err = lib().dro_dream_arr_rect(
-        _p(Ro), No, a, b, dx, dy, dt, int(nt), dtype, _p(delay), v, cp, L, _p(G),
-        FOC[foc_met], _p(focal), STEER[steer_met], theta, phi, _p(apod_v), int(do_apod),
-        APOD.get(apod_met, 0), apod_par, _p(H), ERR_LEVEL[err_level], int(nthreads),
-        ctypes.byref(oob))
+    with _alpha(alpha):
+        err = lib().dro_dream_arr_rect(
+            _p(Ro), No, a, b, dx, dy, dt, int(nt), dtype, _p(delay), v, cp, L, _p(G),
+            FOC[foc_met], _p(focal), STEER[steer_met], theta, phi, _p(apod_v), int(do_apod),
+            APOD.get(apod_met, 0), apod_par, _p(H), ERR_LEVEL[err_level], int(nthreads),
+            ctypes.byref(oob))
     return H, err, oob.value
 
 
-def dreamcirc(Ro, R, dx, dy, dt, nt, delay, v, cp, err_level="stop", nthreads=0):
+def dreamcirc(Ro, R, dx, dy, dt, nt, delay, v, cp, err_level="stop", nthreads=0, alpha=0.0):
     """Circ::dreamcirc (no attenuation).  Returns (H, err code, oob count)."""
     Ro = _f64(np.atleast_2d(Ro))
     No = Ro.shape[0]
     delay = _f64(np.atleast_1d(np.asarray(delay, dtype=np.float64).ravel()))
     H = np.zeros((int(nt), No), dtype=np.float64, order="F")
     oob = ctypes.c_longlong(0)
-    err = lib().dro_dreamcirc(_p(Ro), No, R, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1,
-                              _p(delay), v, cp, _p(H), ERR_LEVEL[err_level], int(nthreads),
-                              ctypes.byref(oob))
+    with _alpha(alpha):
+        err = lib().dro_dreamcirc(_p(Ro), No, R, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1,
+                                  _p(delay), v, cp, _p(H), ERR_LEVEL[err_level], int(nthreads),
+                                  ctypes.byref(oob))
     return H, err, oob.value
 
 
 def dream_arr_circ(Ro, R, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=0.0,
                    steer_met="off", theta=0.0, phi=0.0, apod_met="off", apod=None,
-                   apod_par=0.0, err_level="stop", nthreads=0):
+                   apod_par=0.0, err_level="stop", nthreads=0, alpha=0.0):
     """ArrCirc::dream_arr_circ (no attenuation).  Returns (H, err code, oob count)."""
     Ro = _f64(np.atleast_2d(Ro))
     G = _f64(np.atleast_2d(G))
@@ -262,30 +281,32 @@ def dream_arr_circ(Ro, R, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=
     apod_v = _f64(np.ones(L) if apod is None else np.asarray(apod, dtype=np.float64).ravel())
     H = np.zeros((int(nt), No), dtype=np.float64, order="F")
     oob = ctypes.c_longlong(0)
-    err = lib().dro_dream_arr_circ(
-        _p(Ro), No, R, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1, _p(delay), v, cp, L,
-        _p(G), FOC[foc_met], _p(focal), STEER[steer_met], theta, phi, _p(apod_v), int(do_apod),
-        APOD.get(apod_met, 0), apod_par, _p(H), ERR_LEVEL[err_level], int(nthreads),
-        ctypes.byref(oob))
+    with _alpha(alpha):
+        err = lib().dro_dream_arr_circ(
+            _p(Ro), No, R, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1, _p(delay), v, cp, L,
+            _p(G), FOC[foc_met], _p(focal), STEER[steer_met], theta, phi, _p(apod_v), int(do_apod),
+            APOD.get(apod_met, 0), apod_par, _p(H), ERR_LEVEL[err_level], int(nthreads),
+            ctypes.byref(oob))
     return H, err, oob.value
 
 
-def dreamcylind(Ro, a, b, Rcurv, dx, dy, dt, nt, delay, v, cp, err_level="stop", nthreads=0):
+def dreamcylind(Ro, a, b, Rcurv, dx, dy, dt, nt, delay, v, cp, err_level="stop", nthreads=0, alpha=0.0):
     """Cylind::dreamcylind (no attenuation).  Returns (H, err code, oob count)."""
     Ro = _f64(np.atleast_2d(Ro))
     No = Ro.shape[0]
     delay = _f64(np.atleast_1d(np.asarray(delay, dtype=np.float64).ravel()))
     H = np.zeros((int(nt), No), dtype=np.float64, order="F")
     oob = ctypes.c_longlong(0)
-    err = lib().dro_dreamcylind(_p(Ro), No, a, b, Rcurv, dx, dy, dt, int(nt),
-                                0 if delay.size == 1 else 1, _p(delay), v, cp, _p(H),
-                                ERR_LEVEL[err_level], int(nthreads), ctypes.byref(oob))
+    with _alpha(alpha):
+        err = lib().dro_dreamcylind(_p(Ro), No, a, b, Rcurv, dx, dy, dt, int(nt),
+                                    0 if delay.size == 1 else 1, _p(delay), v, cp, _p(H),
+                                    ERR_LEVEL[err_level], int(nthreads), ctypes.byref(oob))
     return H, err, oob.value
 
 
 def dream_arr_cylind(Ro, a, b, Rcurv, dx, dy, dt, nt, delay, v, cp, G, foc_met="off",
                      focal=0.0, steer_met="off", theta=0.0, phi=0.0, apod_met="off", apod=None,
-                     apod_par=0.0, err_level="stop", nthreads=0):
+                     apod_par=0.0, err_level="stop", nthreads=0, alpha=0.0):
     """ArrCylind::dream_arr_cylind (no attenuation).  Returns (H, err code, oob count)."""
     Ro = _f64(np.atleast_2d(Ro))
     G = _f64(np.atleast_2d(G))
@@ -298,16 +319,17 @@ def dream_arr_cylind(Ro, a, b, Rcurv, dx, dy, dt, nt, delay, v, cp, G, foc_met="
     apod_v = _f64(np.ones(L) if apod is None else np.asarray(apod, dtype=np.float64).ravel())
     H = np.zeros((int(nt), No), dtype=np.float64, order="F")
     oob = ctypes.c_longlong(0)
-    err = lib().dro_dream_arr_cylind(
-        _p(Ro), No, a, b, Rcurv, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1, _p(delay), v,
-        cp, L, _p(G), FOC[foc_met], _p(focal), STEER[steer_met], theta, phi, _p(apod_v),
-        int(do_apod), APOD.get(apod_met, 0), apod_par, _p(H), ERR_LEVEL[err_level],
-        int(nthreads), ctypes.byref(oob))
+    with _alpha(alpha):
+        err = lib().dro_dream_arr_cylind(
+            _p(Ro), No, a, b, Rcurv, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1, _p(delay), v,
+            cp, L, _p(G), FOC[foc_met], _p(focal), STEER[steer_met], theta, phi, _p(apod_v),
+            int(do_apod), APOD.get(apod_met, 0), apod_par, _p(H), ERR_LEVEL[err_level],
+            int(nthreads), ctypes.byref(oob))
     return H, err, oob.value
 
 
 def dream_arr_annu(Ro, Gr, dx, dy, dt, nt, delay, v, cp, foc_met="off", focal=0.0,
-                   apod_met="off", apod=None, apod_par=0.0, err_level="stop", nthreads=0):
+                   apod_met="off", apod=None, apod_par=0.0, err_level="stop", nthreads=0, alpha=0.0):
     """ArrAnnu::dream_arr_annu (no attenuation).  Gr is the vector of radii (centre
     disc, then inner/outer per ring).  Returns (H, err code, oob count)."""
     Ro = _f64(np.atleast_2d(Ro))
@@ -322,10 +344,11 @@ def dream_arr_annu(Ro, Gr, dx, dy, dt, nt, delay, v, cp, foc_met="off", focal=0.
                   else np.asarray(apod, dtype=np.float64).ravel())
     H = np.zeros((int(nt), No), dtype=np.float64, order="F")
     oob = ctypes.c_longlong(0)
-    err = lib().dro_dream_arr_annu(
-        _p(Ro), No, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1, _p(delay), v, cp, nr,
-        _p(Gr), FOC[foc_met], _p(focal), _p(apod_v), int(do_apod), APOD.get(apod_met, 0),
-        apod_par, _p(H), ERR_LEVEL[err_level], int(nthreads), ctypes.byref(oob))
+    with _alpha(alpha):
+        err = lib().dro_dream_arr_annu(
+            _p(Ro), No, dx, dy, dt, int(nt), 0 if delay.size == 1 else 1, _p(delay), v, cp, nr,
+            _p(Gr), FOC[foc_met], _p(focal), _p(apod_v), int(do_apod), APOD.get(apod_met, 0),
+            apod_par, _p(H), ERR_LEVEL[err_level], int(nthreads), ctypes.byref(oob))
     return H, err, oob.value
 
 

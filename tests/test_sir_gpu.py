@@ -167,10 +167,6 @@ def test_error_codes_gpu(gpu):
     _, c3, _ = arr_rect(Rt, 0.9, 10.0, 0.05, 0.05, 0.025, 200, delay, 1.0, 1500.0, G_first,
                         err_level="stop")
     assert c3 == 3
-    with pytest.raises(RuntimeError):
-        dream_arr_rect.dream_arr_rect(Ro, [0.9, 10.0], np.zeros((2, 3)), [0.05, 0.05, 0.025, 100],
-                                      0.0, [1.0, 1500.0, 5.0], "off", 0.0, "off", [0, 0], "off",
-                                      None, 0.0, "stop")   # attenuation: not on this path
 
 
 def test_batch_plan_uneven_with_graph_replays(gpu, oracle):
