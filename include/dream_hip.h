@@ -50,7 +50,7 @@ extern "C" {
 #define DREAM_HIP_EINVAL      (-1)   /* invalid argument                       */
 #define DREAM_HIP_ERUNTIME    (-2)   /* HIP runtime error (no device, fault..)  */
 #define DREAM_HIP_ENOMEM      (-3)   /* device allocation failed               */
-#define DREAM_HIP_EUNSUPPORTED (-4)  /* feature not on this path (attenuation)  */
+#define DREAM_HIP_EUNSUPPORTED (-4)  /* feature not supported on this path      */
 #define DREAM_HIP_ECANCELLED  (-5)   /* dream_hip_abort() was called           */
 
 /* Output / data precision selectors. */
@@ -95,8 +95,11 @@ int dream_hip_is_running(void);
 
 /* dream_arr_rect on the GPU.  Replaces ArrRect::dream_arr_rect
  * (include/dream_arr_rect.h:40-51, src/dream_arr_rect.cc:310-433): same
- * arguments in the same order.  alpha must be 0 (attenuation is not on this
- * path: DREAM_HIP_EUNSUPPORTED otherwise).  Host pointers.  h (nt x No FP64)
+ * arguments in the same order.  alpha [dB/(cm MHz)] > DBL_EPSILON adds the
+ * reference's frequency-dependent attenuation (Attenuation::att,
+ * src/attenuation.cc:72-140): every in-range contribution becomes a lossy
+ * pulse, computed per point in the frequency domain and transformed once
+ * (nt at most 7422).  Host pointers.  h (nt x No FP64)
  * is overwritten (zeroed, then accumulated), which equals the reference's
  * accumulate-into-a-zeroed-buffer contract.
  * *sir_err: for err_level stop, out_of_bounds if any contribution fell
@@ -144,7 +147,7 @@ int dream_hip_arr_rect_dev(const double *Ro, int64_t No, double a, double b, dou
  * the DR sum walks rows ys = -R + dy/2 (+= dy, while ys <= R) and in each
  * row xs = -sqrt(R*R - ys*ys) + dx/2 (+= dx, while xs <= sqrt(R*R - ys*ys)),
  * with the index and amplitude rules of the rectangle.  Host pointers;
- * FP64 output; alpha must be 0. */
+ * FP64 output; alpha as for dream_hip_arr_rect. */
 int dream_hip_arr_circ(double alpha, const double *Ro, int64_t No, double R, double dx,
                        double dy, double dt, int64_t nt, int delay_type, const double *delay,
                        double v, double cp, int64_t num_elements, const double *G, int foc_met,
@@ -174,7 +177,7 @@ int dream_hip_arr_circ_dev(const double *Ro, int64_t No, double R, double dx, do
  * sqrt(Rcurv^2 - ys^2) (rz = z - zs focused, z + zs defocused), and in each
  * row the rectangle's x walk; the surface element is |Rcurv| dx dphi.  With
  * b > 2|Rcurv| the reference's angles are not numbers and the response is
- * zero.  Host pointers; FP64 output; alpha must be 0. */
+ * zero.  Host pointers; FP64 output; alpha as for dream_hip_arr_rect. */
 int dream_hip_arr_cylind(double alpha, const double *Ro, int64_t No, double a, double b,
                          double Rcurv, double dx, double dy, double dt, int64_t nt,
                          int delay_type, const double *delay, double v, double cp,
@@ -208,7 +211,8 @@ int dream_hip_arr_cylind_dev(const double *Ro, int64_t No, double a, double b, d
  * apod are host arrays (focal and apod: num_radii values for 'ud', of which
  * the first (num_radii+1)/2 are used).  *sir_err: E_OOB if any disc of any
  * point stopped; otherwise the status of the last disc.  Host pointers for
- * Ro, delay and h; FP64 output; alpha must be 0. */
+ * Ro, delay and h; FP64 output; alpha as for dream_hip_arr_rect (every disc is
+ * attenuated, then the rings are formed, as in the reference). */
 int dream_hip_arr_annu(double alpha, const double *Ro, int64_t No, double dx, double dy,
                        double dt, int64_t nt, int delay_type, const double *delay, double v,
                        double cp, int64_t num_radii, const double *Gr, int foc_met,

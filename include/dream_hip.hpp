@@ -16,9 +16,9 @@
 // gateways' `if (!obj.is_running())` error branches fire as before; the
 // flag is set again at the start of every call.  Differences: h is
 // overwritten rather than accumulated into (every reference caller zeroes
-// it first, python/src/py_dream_arr_rect.cc:188-189), attenuation (alpha >
-// 0) throws std::runtime_error, and HIP errors throw std::runtime_error as
-// the reference's OpenCL setup errors do (include/das.h:159-162).
+// it first, python/src/py_dream_arr_rect.cc:188-189), and HIP errors
+// throw std::runtime_error as the reference's OpenCL setup errors do
+// (include/das.h:159-162).  Attenuation (alpha > 0) runs on the GPU.
 #pragma once
 
 #include <stdexcept>
