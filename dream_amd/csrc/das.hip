@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <utility>
@@ -552,9 +553,13 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj)
   bool live[kRPU];
   const int64_t base = region * kRegion;
   GT bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+  // Lanes take their 64-pixel block transposed (lane l -> position
+  // 8 (l mod 8) + l / 8): the visiting order runs along x inside a block,
+  // and consecutive lanes a row apart scatter into different LDS bins.
+  const int lane_pos = (int) ((threadIdx.x & ~63u) | ((threadIdx.x & 7u) << 3) | ((threadIdx.x >> 3) & 7u));
 #pragma unroll
   for (int q = 0; q < kRPU; q++) {
-    const int64_t idx = base + (int64_t) q * kThreads + threadIdx.x;
+    const int64_t idx = base + (int64_t) q * kThreads + lane_pos;
     live[q] = idx < A.No;
     const int64_t p = pixel_of(A, perm, live[q] ? idx : base);
     xo[q] = Ro[p]; yo[q] = Ro[p + A.No]; zo[q] = Ro[p + 2 * A.No];
@@ -1233,6 +1238,20 @@ void pixel_order(const GT *Ro, int64_t No, int32_t *perm)
                      return x.first < y.first;
                    });
   for (int64_t n = 0; n < No; n++) perm[n] = key[(size_t) n].second;
+  // Within each 64-pixel block (one wave), order by (z, y, x): the 4-lane
+  // quads, whose gathers the L1 serves together, then run along x, where the
+  // sample index changes more slowly than along z for arrays in the z = 0
+  // plane (measured on MI355X: C2 TFM 2.5 % and C5 RCA 13 % faster than
+  // Morton quads).
+  for (int64_t b = 0; b + 64 <= No; b += 64) {
+    std::stable_sort(perm + b, perm + b + 64, [&](int32_t u, int32_t v) {
+      for (int c = 2; c >= 0; c--) {
+        const double a = (double) Ro[u + c * No], d = (double) Ro[v + c * No];
+        if (a != d) return a < d;
+      }
+      return false;
+    });
+  }
 }
 
 template void pixel_order<double>(const double *, int64_t, int32_t *);

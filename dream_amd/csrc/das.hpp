@@ -41,7 +41,8 @@ size_t das_forward_scratch_bytes(int type, int prec, int64_t No, int64_t n_frame
 
 // Spatially compact visiting order of the pixels (Morton order of the
 // quantised coordinates), so the 64 pixels of a wave form a compact tile and
-// their gathers share cache lines.  Ro is No x 3 column-major.
+// their gathers share cache lines; inside each 64-pixel tile the pixels are
+// ordered by (z, y, x).  Ro is No x 3 column-major.
 template <typename GT>
 void pixel_order(const GT *Ro, int64_t No, int32_t *perm);
 

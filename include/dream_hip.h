@@ -289,8 +289,9 @@ int dream_hip_das_destroy(void *handle);
 int dream_hip_das_set_pixel_order(void *handle, const int32_t *perm, int64_t n);
 
 /* Spatially compact order of No points (Morton order of the coordinates
- * quantised to 1024 cells per axis over their bounding box).  Ro is No x 3
- * column-major FP64 (HOST); perm (HOST, No entries) receives the order. */
+ * quantised to 1024 cells per axis over their bounding box; inside each run
+ * of 64, ordered by (z, y, x)).  Ro is No x 3 column-major FP64 (HOST); perm
+ * (HOST, No entries) receives the order. */
 int dream_hip_pixel_order(const double *Ro, int64_t No, int32_t *perm);
 
 /* 1 if (das_type, a_scan_len, No, num_t, num_r, precision) differ from the
