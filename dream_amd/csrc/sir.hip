@@ -106,7 +106,7 @@ __device__ __forceinline__ int find_problem(const SirDev *P, int n, int64_t item
 // The run-length sums and the LDS histogram are FP64 for both output
 // precisions: gfx950's LDS FP64 atomic add is about 20 times faster than its
 // FP32 one, and the column is rounded to T once, when it is written.
-template <typename T>
+template <typename T, bool ANY_GLOBAL>
 __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
 {
   const int64_t n = local / P.chunks;
@@ -121,7 +121,8 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
   const int64_t nt = P.nt;
   const double ntd = (double) nt;
   T *out = reinterpret_cast<T *>(P.h) + n * nt;
-  const bool use_lds = P.mode != SIR_GLOBAL;
+  // Plans without a global-mode problem compile the LDS path only.
+  const bool use_lds = !ANY_GLOBAL || P.mode != SIR_GLOBAL;
 
   if (use_lds) {
     for (int i = threadIdx.x; i < nt; i += kThreads) hist[i] = 0.0;
@@ -237,7 +238,7 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
   }
 }
 
-template <typename T>
+template <typename T, bool ANY_GLOBAL>
 __global__ void __launch_bounds__(kThreads)
 sir_kernel(const SirDev *__restrict__ probs, int n_probs, int64_t n_items,
            unsigned long long *counter)
@@ -248,7 +249,7 @@ sir_kernel(const SirDev *__restrict__ probs, int n_probs, int64_t n_items,
     const int64_t item = blockIdx.x;
     if (item >= n_items) return;
     const int p = find_problem(probs, n_probs, item);
-    sir_item<T>(probs[p], item - probs[p].item_begin, hist);
+    sir_item<T, ANY_GLOBAL>(probs[p], item - probs[p].item_begin, hist);
     return;
   }
   // Persistent: pull items from counter[0]; the last work-group to leave
@@ -261,7 +262,7 @@ sir_kernel(const SirDev *__restrict__ probs, int n_probs, int64_t n_items,
     __syncthreads();
     if (item >= n_items) break;
     const int p = find_problem(probs, n_probs, item);
-    sir_item<T>(probs[p], item - probs[p].item_begin, hist);
+    sir_item<T, ANY_GLOBAL>(probs[p], item - probs[p].item_begin, hist);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -708,12 +709,22 @@ void SirPlan::launch_shape(const SirShape &sh, hipStream_t s)
     }
   if (n_items_ == 0) return;
   unsigned long long *ctr = sh.persistent ? d_counter_ : nullptr;
+  bool any_global = false;
+  for (const SirDev &d : host_) any_global |= d.mode == SIR_GLOBAL;
   if (out_dtype_ == DREAM_HIP_F64) {
-    hipLaunchKernelGGL(sir_kernel<double>, dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
-                       d_probs_, (int) host_.size(), n_items_, ctr);
+    if (any_global)
+      hipLaunchKernelGGL((sir_kernel<double, true>), dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
+                         d_probs_, (int) host_.size(), n_items_, ctr);
+    else
+      hipLaunchKernelGGL((sir_kernel<double, false>), dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
+                         d_probs_, (int) host_.size(), n_items_, ctr);
   } else {
-    hipLaunchKernelGGL(sir_kernel<float>, dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
-                       d_probs_, (int) host_.size(), n_items_, ctr);
+    if (any_global)
+      hipLaunchKernelGGL((sir_kernel<float, true>), dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
+                         d_probs_, (int) host_.size(), n_items_, ctr);
+    else
+      hipLaunchKernelGGL((sir_kernel<float, false>), dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
+                         d_probs_, (int) host_.size(), n_items_, ctr);
   }
   DREAM_HIP_CHECK(hipGetLastError());
 }
