@@ -972,6 +972,7 @@ FwdShape forward_shape(int type, int64_t No, int64_t n_frames, int64_t Lt, int64
   const int64_t J = type == DAS_SAFT ? (Lt + kSB - 1) / kSB
                                      : ((Lr + kRC - 1) / kRC) * ((Lt + kTB - 1) / kTB);
   FwdShape sh{4, 1};
+  if (tiles == 0) return sh;                              // no pixels: nothing launches
   if (tiles * 4 < target) sh.waves = kMaxWaves;
   sh.waves = (int) std::max<int64_t>(1, std::min<int64_t>(sh.waves, J));
   const int64_t waves_total = tiles * sh.waves;
