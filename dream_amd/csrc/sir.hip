@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -378,6 +379,55 @@ double cylinder_rows(double b, double Rc, double dy, std::vector<double> &ys,
   return dphi;
 }
 
+void split_disc_rows(int64_t L, double dx, std::vector<double> &ys, std::vector<double> &x0,
+                     std::vector<double> &count)
+{
+  // Rows are dealt one per lane, so a disc whose rows number fewer than the
+  // lanes, or whose rows differ in length, leaves lanes idle.  Cut the rows
+  // into segments of at most S sub-elements, S chosen to minimise (passes of
+  // 256 lanes) x S over the point's L x segments; each segment starts at the
+  // x the row's accumulated walk reaches there, so every position is the
+  // reference's bit for bit.
+  double mx = 0.0;
+  for (double c : count) mx = std::max(mx, c);
+  const int64_t maxc = (int64_t) mx;
+  if (maxc <= 1 || L < 1) return;
+  int64_t best_s = maxc;
+  double best_t = 0.0;
+  // Segments shorter than 48 cost more in per-row set-up than they gain in
+  // balance (measured on MI355X: the annular benchmark's disc launch took
+  // 2.15 ms unsplit, 1.83 ms with segments down to 9 sub-elements, and the
+  // whole call was fastest with a floor of 48).
+  const int64_t smin = std::max<int64_t>({(int64_t) 48, maxc / 16});
+  for (int64_t S = maxc; S >= smin; S = std::min(S - 1, S * 31 / 32)) {
+    int64_t q = 0;
+    for (double c : count) q += ((int64_t) c + S - 1) / S;
+    const double t = std::ceil((double) (q * L) / (double) kThreads) * (double) S;
+    if (S == maxc || t < best_t) { best_t = t; best_s = S; }
+  }
+  if (best_s == maxc) return;
+  std::vector<double> vy, vx, vn;
+  for (size_t r = 0; r < ys.size(); r++) {
+    double x = x0[r];
+    int64_t left = (int64_t) count[r];
+    if (left == 0) {
+      vy.push_back(ys[r]); vx.push_back(x); vn.push_back(0.0);
+      continue;
+    }
+    while (left > 0) {
+      const int64_t c = std::min(best_s, left);
+      vy.push_back(ys[r]);
+      vx.push_back(x);
+      vn.push_back((double) c);
+      for (int64_t k = 0; k < c; k++) x += dx;
+      left -= c;
+    }
+  }
+  ys.swap(vy);
+  x0.swap(vx);
+  count.swap(vn);
+}
+
 std::vector<double> subgrid(double w, double d)
 {
   std::vector<double> v;
@@ -541,6 +591,7 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
     int64_t nx_cost;
     if (p.shape == SIR_CIRC) {
       disc_rows(p.R, p.dx, p.dy, ys[i], rx0[i], rn[i]);
+      split_disc_rows((int64_t) (p.G.size() / 3), p.dx, ys[i], rx0[i], rn[i]);
       double sum = 0.0, mx = 0.0;
       for (double c : rn[i]) { sum += c; mx = std::max(mx, c); }
       xs[i].assign(1, 0.0);                     // unused by the disc walk

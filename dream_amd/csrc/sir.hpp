@@ -58,6 +58,11 @@ enum { SIR_RECT = 0, SIR_CIRC = 1, SIR_CYLIND = 2 };
 void disc_rows(double R, double dx, double dy, std::vector<double> &ys,
                std::vector<double> &x0, std::vector<double> &count);
 
+// Cuts disc rows into segments of equal-ish length for lane balance
+// (tables as disc_rows gives them; L elements per point).
+void split_disc_rows(int64_t L, double dx, std::vector<double> &ys, std::vector<double> &x0,
+                     std::vector<double> &count);
+
 // Sub-element rows of a cylindrically curved element of y-size b and
 // curvature radius Rc (src/dreamcylind.cc:288-350): the rows are angles phi
 // from -phi_max + dphi/2 in steps dphi = asin(dy/|Rc|) while phi <= phi_max,
