@@ -12,8 +12,10 @@ with the data already resident in HBM.  value = pixel-channel sums per second
   cpu_baseline  the reference's own DAS<float> (das.cc compiled from source,
                 oracle/_ref) or, where that is absent, the oracle's
                 restatement, on the same image on the host cores;
-  extra         the other measured paths (FP64 / reference-float TFM, C1 SIR,
-                C3 batched SIR, C4 SAFT forward/backward, C5 RCA frames).
+  extra         the other measured paths (FP64 / reference-float TFM, the C2
+                image through the host-buffer entry point with its PCIe
+                copies, a 256 MiB device copy, C1 SIR, C3 batched SIR, C4
+                SAFT forward/backward, C5 RCA frames).
 Only one GPU is used; --gpus N > 1 exits with a message.
 """
 import argparse
@@ -103,10 +105,44 @@ def bench_c2(steps, warmup, want_cpu, want_extra=True):
         m = time_launches(lambda: p.forward(Yp, w["dt"], 0.0, w["cp"], out=o), steps, warmup)
         extra[f"c2_tfm_{prec}"] = dict(ms=round(m, 4), sums_per_s=n_pix * n_ch / (m * 1e-3))
         del Yp
+    if want_extra:
+        extra["c2_tfm_f32_host_call"] = c2_host_call(w, Y, n_pix, n_ch)
+        extra["device_copy_256MiB"] = device_copy_rate(steps, warmup)
     cpu = None
     if want_cpu:
         cpu = cpu_baseline_c2(w, n_pix, n_ch)
     return res, extra, cpu
+
+
+def c2_host_call(w, Y, n_pix, n_ch, calls=10):
+    """The C2 image through the host-buffer entry point (the reference's
+    das() contract: Y, Ro, G in host memory, Im returned to the host), so the
+    time includes the copy of the 33.5 MB of A-scans over PCIe and the image
+    back.  Median of `calls` calls after one warm-up."""
+    import numpy as np
+    from dream_amd import das_f
+
+    Yh = Y.cpu().numpy().T                 # K x n_ascans, Fortran order
+    args = (Yh, w["Gt"], w["Gr"], w["Ro"], w["dt"], 0.0, w["cp"], "tfm", "ignore")
+    das_f.das(*args)
+    ts = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        das_f.das(*args)
+        ts.append(time.perf_counter() - t0)
+    ms = float(np.median(ts)) * 1e3
+    return dict(ms=round(ms, 3), sums_per_s=n_pix * n_ch / (ms * 1e-3),
+                host_bytes_in=int(Yh.nbytes), calls=calls)
+
+
+def device_copy_rate(steps, warmup):
+    """Measured device-to-device copy rate (256 MiB read + 256 MiB written per
+    copy), beside the 8 TB/s HBM peak the roofline is priced against."""
+    import torch
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    ms = time_launches(lambda: dst.copy_(src), steps, warmup)
+    return dict(ms=round(ms, 4), gb_per_s=2 * src.numel() * 4 / (ms * 1e-3) / 1e9)
 
 
 def _sir_device_problem(p, torch):
@@ -339,8 +375,13 @@ def bench_das_diff_c4(steps, warmup):
     bwd = time_launches(lambda: plan.adjoint(Y, g, w["dt"], 0.0, w["cp"], apod, None, "linear"),
                         steps, warmup)
     pairs = plan.No * plan.n_ascans
+    # In-bounds pairs (nearest-lower index in [0, K)): all-ones data, exact sums.
+    n_inb = int(plan.forward(torch.ones_like(Y), w["dt"], 0.0, w["cp"]).double().sum().item())
+    alg_bytes = n_inb * 2 * 4 + plan.No * 4              # two taps per pair, FP32
     out["c4_saft_linear_f32"] = dict(fwd_ms=round(fwd, 4), bwd_ms=round(bwd, 4),
-                                     bwd_over_fwd=bwd / fwd, sums_per_s=pairs / (fwd * 1e-3))
+                                     bwd_over_fwd=bwd / fwd, sums_per_s=pairs / (fwd * 1e-3),
+                                     in_bounds_pairs=n_inb,
+                                     roofline_frac=alg_bytes / (fwd * 1e-3) / HBM_PEAK)
     return out
 
 
@@ -362,8 +403,12 @@ def bench_das_diff_c5(steps, warmup):
     bwd = time_launches(lambda: plan.adjoint(Y, g, w["dt"], 0.0, w["cp"], ones, ones, "nearest",
                                              want_gy=False, want_cp=False), 2, 1)
     pairs = plan.No * plan.n_ascans * F
+    n_inb = int(plan.forward(torch.ones_like(Y), w["dt"], 0.0, w["cp"]).double().sum().item())
+    alg_bytes = n_inb * 4 + F * plan.No * 4
     out["c5_rca_8frames_f32"] = dict(fwd_ms=round(fwd, 3), bwd_apod_ms=round(bwd, 3),
-                                     bwd_over_fwd=bwd / fwd, sums_per_s=pairs / (fwd * 1e-3))
+                                     bwd_over_fwd=bwd / fwd, sums_per_s=pairs / (fwd * 1e-3),
+                                     in_bounds_pairs=n_inb,
+                                     roofline_frac=alg_bytes / (fwd * 1e-3) / HBM_PEAK)
     return out
 
 

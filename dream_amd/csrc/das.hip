@@ -34,6 +34,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -183,6 +184,42 @@ __device__ __forceinline__ DT fetch_ascan(__amdgpu_buffer_rsrc_t r, unsigned aba
   }
 }
 
+// SAFT sample position u = 2 (sqrt(s) / cp + delay) fs, with the rounding
+// of the IEEE chain (extra_src/das.cc:329-335).  FAST (FP64 geometry only):
+// sqrt(s) from the rsqrt seed and one Goldschmidt step, relative error below
+// 2^-47.5 (the seed is accurate to 2^-24.2 on gfx950).  Carried through the
+// multiply, the delay add, the doubling, the clamp (1-Lipschitz) and the
+// multiply by fs, the fast u differs from the IEEE one by less than
+// 2^-46 (|u| + |2 fs delay| + 1); when it lies farther than 64 times that
+// from an integer, trunc and floor of the two agree and it is kept, else
+// (about one pair in 10^8, and for s = 0 or non-finite s) the IEEE chain is
+// recomputed.  `bnd` = 2^-40 (|2 fs delay| + 1), per pixel.  The kept value
+// moves the linear-interpolation weight by at most 2^-40 |u|, so FAST is only
+// used where that is far below the output's precision (nearest-lower
+// sampling, or FP32 data).  `tleg` returns the leg time sqrt(s) / cp.
+template <typename GT, bool FAST>
+__device__ __forceinline__ GT saft_pos(GT dx, GT dy, GT dz, GT inv_cp, GT dl, GT tmax, GT fs,
+                                       GT bnd, GT &tleg)
+{
+  if (FAST) {
+    const double s = (dx * dx + dy * dy) + dz * dz;
+    double r, rinv;
+    sqrt_rsqrt<1>(s, r, rinv);
+    tleg = r * inv_cp;
+    const double t = tleg + dl;
+    const double u = clamp_leg<double>(t + t, tmax) * fs;
+    if (__builtin_fabs(u - __builtin_rint(u)) > __builtin_fma(__builtin_fabs(u), 0x1p-40, bnd))
+      return u;
+  }
+  tleg = leg3<GT>(dx, dy, dz) * inv_cp;
+  const GT t = tleg + dl;
+  return clamp_leg<GT>(t + t, tmax) * fs;     // (T)(2.0*t) * fs, 2t exact
+}
+
+template <typename DT, typename GT, int INTERP>
+constexpr bool kSaftFast = std::is_same<GT, double>::value &&
+                           (INTERP == DREAM_HIP_NEAREST || sizeof(DT) == 4);
+
 // --------------------------------------------------------------------- forward
 
 // Work of a 64-pixel tile is cut into jobs: (receive chunk of kRC, transmit
@@ -224,6 +261,7 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
   DT acc = DT(0);
 
   if (TYPE == DAS_SAFT) {
+    const GT bnd = (GT) (0x1p-40 * (fabs(2.0 * (double) fs * (double) dl) + 1.0));
     const int J = (Lt + kSB - 1) / kSB;
     for (int job = job0; job < J; job += jstride) {
       const int n0 = job * kSB;
@@ -232,9 +270,10 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
 #pragma unroll
       for (int q = 0; q < kSB; q++) {
         const int n = min(n0 + q, Lt - 1);
-        GT t = leg3<GT>(Gt[n] - xo, Gt[n + Lt] - yo, Gt[n + 2 * Lt] - zo) * inv_cp;
-        t += dl;
-        u[q] = clamp_leg<GT>(t + t, tmax) * fs;    // (T)(2.0*t) * fs, 2t exact
+        GT tleg;
+        u[q] = saft_pos<GT, kSaftFast<DT, GT, INTERP>>(Gt[n] - xo, Gt[n + Lt] - yo,
+                                                      Gt[n + 2 * Lt] - zo, inv_cp, dl, tmax,
+                                                      fs, bnd, tleg);
         any_in |= (n0 + q < Lt) && u[q] > GT(-1) && u[q] < (GT) K;
       }
       if (!__any(any_in && valid)) {               // the whole block is out of range
@@ -539,6 +578,7 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj)
   const GT *Gr = (const GT *) A.Gr;
   const GT fs = (GT) A.fs_khz, inv_cp = (GT) A.inv_cp, dl = (GT) A.delay_ms;
   const GT tmax = (GT) A.leg_max;
+  const GT bnd = (GT) (0x1p-40 * (fabs(2.0 * (double) fs * (double) dl) + 1.0));
   const DT *g = (const DT *) A.g + frame * A.im_stride;
   const DT *at = (const DT *) A.apod_t;
   const DT *ar = (const DT *) A.apod_r;
@@ -685,10 +725,11 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj)
 #pragma unroll
     for (int q = 0; q < kRPU; q++) {
       if (TYPE == DAS_SAFT) {
-        GT t = leg3<GT>(Gt[a] - xo[q], Gt[a + Lt] - yo[q], Gt[a + 2 * Lt] - zo[q]) * inv_cp;
-        tsum[q] = t + t;
-        t += dl;
-        u[q] = clamp_leg<GT>(t + t, tmax) * fs;
+        GT tleg;
+        u[q] = saft_pos<GT, kSaftFast<DT, GT, INTERP>>(Gt[a] - xo[q], Gt[a + Lt] - yo[q],
+                                                      Gt[a + 2 * Lt] - zo[q], inv_cp, dl, tmax,
+                                                      fs, bnd, tleg);
+        tsum[q] = tleg + tleg;
       } else {
         const int i = min(i0 + ii, Lt - 1);
         GT tt;

@@ -81,6 +81,40 @@ def test_ramp_indices_exact_all_precisions(gpu, oracle, method):
         assert np.array_equal(Im.cpu().numpy().astype(np.float64), ref64)
 
 
+@pytest.mark.parametrize("delay", [0.0, 0.37])
+def test_saft_positions_on_sample_boundaries(gpu, oracle, delay):
+    """Pixels straight below one element whose sample position 2 (z/cp +
+    delay) fs lies within 1e-13 .. 1e-7 of an integer, on both sides: the
+    kernel's fast square root is certified there or replaced by the IEEE one,
+    so the FP64 indices (ramp data) must come out bit for bit, for FP64 data
+    and for FP32 data with FP64 delays."""
+    from dream_amd import das
+    from dream_amd.torch_ops import DasPlan
+    dt, cp, K = 0.02, 1500.0, 4000
+    fs = 1000.0 / dt
+    Gt = np.zeros((1, 3))
+    z = []
+    for n in np.arange(200, 3800, 240):
+        z0 = (n / (2.0 * fs) - delay / 1000.0) * cp
+        z += [z0 * (1.0 + m * 2.0 ** -40) for m in range(-64, 65)]
+        z += [z0 * (1.0 + m * 2.0 ** -52) for m in range(-20, 21)]
+    z = np.array(z)
+    Ro = np.c_[np.zeros_like(z), np.zeros_like(z), z]
+    ramp = np.arange(K, dtype=np.float64)[:, None]
+    ref, _ = oracle.das(ramp, Gt, Gt, Ro, dt, delay, cp, "saft", "ignore", "f64")
+    assert np.unique(np.floor(ref)).size >= 2 * 15        # both sides of every boundary
+    assert np.array_equal(das.das(ramp, Gt, Gt, Ro, dt, delay, cp, "saft", "ignore", "gpu"), ref)
+    plan = DasPlan("saft", K, Ro, Gt, None, precision="f32")
+    Yt = torch.tensor(ramp.T.copy(), dtype=torch.float32, device="cuda")
+    Im = plan.forward(Yt, dt, delay, cp).cpu().numpy().astype(np.float64)
+    assert np.array_equal(Im, ref)
+    # Linear interpolation of a ramp returns the position itself, here within
+    # 1e-7 of an integer (FP32 output: within 1e-3).
+    lin = plan.forward(Yt, dt, delay, cp, interp="linear").cpu().numpy().astype(np.float64)
+    assert np.max(np.abs(lin - np.rint(lin))) <= 1e-3
+    assert np.max(np.abs(lin - ref)) <= 1.0 + 1e-3
+
+
 @pytest.mark.parametrize("method", METHODS)
 def test_random_data_f64_and_error_codes(gpu, oracle, method):
     from dream_amd import das
