@@ -129,86 +129,33 @@ __device__ __forceinline__ GT clamp_leg(GT t, GT tmax)
   return fmin(fmax(t, -tmax), tmax);
 }
 
-// One pair: the value of A-scan `abase` (element offset a*K) at sample
-// position u under the chosen rule; `inb` reports whether the nearest-lower
-// index lies in [0, K) (the reference's bounds test).
-template <typename DT, typename GT, int INTERP>
-__device__ __forceinline__ DT fetch(__amdgpu_buffer_rsrc_t r, unsigned abase, unsigned K, GT u,
-                                    bool live, bool &inb)
-{
-  if (INTERP == DREAM_HIP_NEAREST) {
-    const int k = (int) u;                       // trunc toward zero, u in [-2^27, 2^27]
-    inb = (unsigned) k < K;
-    const unsigned off = (inb && live) ? (abase + (unsigned) k) * (unsigned) sizeof(DT) : kNoSample;
-    return bload<DT>(r, off);
-  } else {
-    const GT fk = floor(u);
-    const int k = (int) fk;
-    inb = (unsigned) k < K;
-    const bool in1 = (unsigned) (k + 1) < K;
-    const unsigned o0 = (inb && live) ? (abase + (unsigned) k) * (unsigned) sizeof(DT) : kNoSample;
-    const unsigned o1 = (in1 && live) ? (abase + (unsigned) (k + 1)) * (unsigned) sizeof(DT) : kNoSample;
-    const DT y0 = bload<DT>(r, o0);
-    const DT y1 = bload<DT>(r, o1);
-    const DT f = (DT) (u - fk);
-    return y0 + f * (y1 - y0);
-  }
-}
-
-// Forward gathers go through one buffer descriptor per frame.  The lane's
-// byte offset is (a*K + k) * sizeof(DT) when the sample index k lies in
-// [0, K), and kNoSample (past the frame, read as 0 by the hardware range
-// check) otherwise: one compare and one select per gather, and the whole
-// launch shares one descriptor (per-A-scan descriptors would need four
-// scalar registers each and spill when a row keeps 16 loads in flight).
-// Legs are clamped (leg_max = 2^26 / fs) so |u| <= 2^27 and the integer
-// conversion is exact.
-template <typename DT, typename GT, int INTERP>
-__device__ __forceinline__ DT fetch_ascan(__amdgpu_buffer_rsrc_t r, unsigned abase, unsigned K,
-                                          GT u, int &k)
-{
-  constexpr unsigned s = (unsigned) sizeof(DT);
-  if (INTERP == DREAM_HIP_NEAREST) {
-    k = (int) u;                                // trunc toward zero
-    const unsigned off = (unsigned) k < K ? (abase + (unsigned) k) * s : kNoSample;
-    return bload<DT>(r, off);
-  } else {
-    const GT fk = floor(u);
-    k = (int) fk;
-    const unsigned o0 = (unsigned) k < K ? (abase + (unsigned) k) * s : kNoSample;
-    const unsigned o1 = (unsigned) (k + 1) < K ? (abase + (unsigned) (k + 1)) * s : kNoSample;
-    const DT y0 = bload<DT>(r, o0);
-    const DT y1 = bload<DT>(r, o1);
-    const DT f = (DT) (u - fk);
-    return y0 + f * (y1 - y0);
-  }
-}
-
 // SAFT sample position u = 2 (sqrt(s) / cp + delay) fs, with the rounding
 // of the IEEE chain (extra_src/das.cc:329-335).  FAST (FP64 geometry only):
-// sqrt(s) from the rsqrt seed and one Goldschmidt step, relative error below
-// 2^-47.5 (the seed is accurate to 2^-24.2 on gfx950).  Carried through the
-// multiply, the delay add, the doubling, the clamp (1-Lipschitz) and the
-// multiply by fs, the fast u differs from the IEEE one by less than
-// 2^-46 (|u| + |2 fs delay| + 1); when it lies farther than 64 times that
-// from an integer, trunc and floor of the two agree and it is kept, else
-// (about one pair in 10^8, and for s = 0 or non-finite s) the IEEE chain is
-// recomputed.  `bnd` = 2^-40 (|2 fs delay| + 1), per pixel.  The kept value
-// moves the linear-interpolation weight by at most 2^-40 |u|, so FAST is only
-// used where that is far below the output's precision (nearest-lower
-// sampling, or FP32 data).  `tleg` returns the leg time sqrt(s) / cp.
+// s by two FMAs and sqrt(s) from the rsqrt seed and one Goldschmidt step,
+// relative error below 2^-47.5 (the seed is accurate to 2^-24.2 on gfx950),
+// then u = (r / cp + delay) (2 fs) (doubling is exact, so this is the
+// chain's own rounding of its operands).  Carried through, the fast u
+// differs from the IEEE one by less than 2^-46 (|u| + |2 fs delay| + 1).
+// It is kept when it lies farther than 2^-29 |u| + `bnd` from an integer,
+// `bnd` = 2^-40 (|2 fs delay| + 1) per pixel: then trunc and floor of the
+// two agree, and |u| < 2^28, so the integer conversion is exact with no
+// clamp.  Otherwise (about one pair in 10^5, and for s = 0 or non-finite
+// s) the IEEE chain is recomputed, clamped as everywhere else.  The kept
+// value moves the linear-interpolation weight by at most 2^-40 |u|, so FAST
+// is only used where that is far below the output's precision
+// (nearest-lower sampling, or FP32 data).  `tleg` returns the leg time
+// sqrt(s) / cp.
 template <typename GT, bool FAST>
 __device__ __forceinline__ GT saft_pos(GT dx, GT dy, GT dz, GT inv_cp, GT dl, GT tmax, GT fs,
-                                       GT bnd, GT &tleg)
+                                       GT fs2, GT bnd, GT &tleg)
 {
   if (FAST) {
-    const double s = (dx * dx + dy * dy) + dz * dz;
+    const double s = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
     double r, rinv;
     sqrt_rsqrt<1>(s, r, rinv);
     tleg = r * inv_cp;
-    const double t = tleg + dl;
-    const double u = clamp_leg<double>(t + t, tmax) * fs;
-    if (__builtin_fabs(u - __builtin_rint(u)) > __builtin_fma(__builtin_fabs(u), 0x1p-40, bnd))
+    const double u = (tleg + dl) * fs2;
+    if (__builtin_fabs(u - __builtin_rint(u)) > __builtin_fma(__builtin_fabs(u), 0x1p-29, bnd))
       return u;
   }
   tleg = leg3<GT>(dx, dy, dz) * inv_cp;
@@ -219,6 +166,35 @@ __device__ __forceinline__ GT saft_pos(GT dx, GT dy, GT dz, GT inv_cp, GT dl, GT
 template <typename DT, typename GT, int INTERP>
 constexpr bool kSaftFast = std::is_same<GT, double>::value &&
                            (INTERP == DREAM_HIP_NEAREST || sizeof(DT) == 4);
+
+// Forward gathers go through one buffer descriptor per frame.  The lane's
+// byte offset is (a*K + k) * sizeof(DT) when the sample index k lies in
+// [0, K), and kNoSample (past the frame, read as 0 by the hardware range
+// check) otherwise: one compare and one select per gather, and the whole
+// launch shares one descriptor (per-A-scan descriptors would need four
+// scalar registers each and spill when a row keeps 16 loads in flight).
+// Positions arrive with |u| < 2^28 (legs clamped to leg_max = 2^26 / fs, or
+// a certified SAFT position), so the integer conversion is exact.  A pair
+// that is not `live` (padding past the last
+// element of a block) reads 0 through the same unconditional load, so the
+// loads of a block never wait on a branch.  Offsets (and the linear weight)
+// come first; callers issue a block's loads together and combine after.
+template <typename DT, typename GT, int INTERP>
+__device__ __forceinline__ void gather_offsets(unsigned abase, unsigned K, GT u, bool live,
+                                               unsigned &o0, unsigned &o1, DT &f, int &k)
+{
+  constexpr unsigned s = (unsigned) sizeof(DT);
+  if (INTERP == DREAM_HIP_NEAREST) {
+    k = (int) u;
+    o0 = (live && (unsigned) k < K) ? (abase + (unsigned) k) * s : kNoSample;
+  } else {
+    const GT fk = floor(u);
+    k = (int) fk;
+    o0 = (live && (unsigned) k < K) ? (abase + (unsigned) k) * s : kNoSample;
+    o1 = (live && (unsigned) (k + 1) < K) ? (abase + (unsigned) (k + 1)) * s : kNoSample;
+    f = (DT) (u - fk);
+  }
+}
 
 // --------------------------------------------------------------------- forward
 
@@ -262,6 +238,7 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
 
   if (TYPE == DAS_SAFT) {
     const GT bnd = (GT) (0x1p-40 * (fabs(2.0 * (double) fs * (double) dl) + 1.0));
+    const GT fs2 = fs + fs;
     const int J = (Lt + kSB - 1) / kSB;
     for (int job = job0; job < J; job += jstride) {
       const int n0 = job * kSB;
@@ -273,26 +250,35 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
         GT tleg;
         u[q] = saft_pos<GT, kSaftFast<DT, GT, INTERP>>(Gt[n] - xo, Gt[n + Lt] - yo,
                                                       Gt[n + 2 * Lt] - zo, inv_cp, dl, tmax,
-                                                      fs, bnd, tleg);
+                                                      fs, fs2, bnd, tleg);
         any_in |= (n0 + q < Lt) && u[q] > GT(-1) && u[q] < (GT) K;
       }
       if (!__any(any_in && valid)) {               // the whole block is out of range
         if (STATUS) oob = true;
         continue;
       }
-      DT v[kSB];
+      // All loads of the block are issued before any is used.
+      unsigned o0[kSB], o1[kSB];
+      DT f[kSB], y0[kSB], y1[kSB];
 #pragma unroll
       for (int q = 0; q < kSB; q++) {
         const int n = min(n0 + q, Lt - 1);
         const bool live = n0 + q < Lt;
         int k;
-        v[q] = live ? fetch_ascan<DT, GT, INTERP>(rY, (unsigned) n * Ku, Ku, u[q], k) : DT(0);
-        if (!live) k = 0;
+        gather_offsets<DT, GT, INTERP>((unsigned) n * Ku, Ku, u[q], live, o0[q], o1[q], f[q], k);
         if (STATUS) oob |= live && (unsigned) k >= Ku;
+      }
+#pragma unroll
+      for (int q = 0; q < kSB; q++) {
+        y0[q] = bload<DT>(rY, o0[q]);
+        if (INTERP != DREAM_HIP_NEAREST) y1[q] = bload<DT>(rY, o1[q]);
       }
       DT part = DT(0);
 #pragma unroll
-      for (int q = 0; q < kSB; q++) part += APOD ? at[min(n0 + q, Lt - 1)] * v[q] : v[q];
+      for (int q = 0; q < kSB; q++) {
+        const DT v = INTERP == DREAM_HIP_NEAREST ? y0[q] : y0[q] + f[q] * (y1[q] - y0[q]);
+        part += APOD ? at[min(n0 + q, Lt - 1)] * v : v;
+      }
       acc += part;
     }
   } else {
@@ -343,33 +329,37 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
           continue;
         }
         const DT wi = APOD ? at[i] : DT(1);
-        // All kRC gathers of the block are issued before any is consumed.
-        DT v[kRC];
-        if (full) {
-#pragma unroll
-          for (int jj = 0; jj < kRC; jj++) {
-            const int j = j0 + jj;
-            const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
-            int k;
-            v[jj] = fetch_ascan<DT, GT, INTERP>(rY, (unsigned) a * Ku, Ku, (tt + tr[jj]) * fs, k);
-            if (STATUS) oob |= (unsigned) k >= Ku;
-          }
-        } else {
-#pragma unroll
-          for (int jj = 0; jj < kRC; jj++) {
-            const int j = min(j0 + jj, Lr - 1);
-            const bool live = j0 + jj < Lr;
-            const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
-            int k;
-            v[jj] = live ? fetch_ascan<DT, GT, INTERP>(rY, (unsigned) a * Ku, Ku, (tt + tr[jj]) * fs, k)
-                         : DT(0);
-            if (!live) k = 0;
-            if (STATUS) oob |= live && (unsigned) k >= Ku;
-          }
-        }
+        // Receivers in groups of kG (the whole chunk for nearest-lower
+        // sampling, half of it for linear interpolation): every load of a
+        // group is issued before any is used.  Receivers past Lr (in the
+        // last chunk) read 0.
+        constexpr int kG = INTERP == DREAM_HIP_NEAREST ? kRC : kRC / 2;
         DT part = DT(0);
 #pragma unroll
-        for (int jj = 0; jj < kRC; jj++) part += APOD ? ar[min(j0 + jj, Lr - 1)] * v[jj] : v[jj];
+        for (int g0 = 0; g0 < kRC; g0 += kG) {
+          unsigned o0[kG], o1[kG];
+          DT f[kG], y0[kG], y1[kG];
+#pragma unroll
+          for (int jj = 0; jj < kG; jj++) {
+            const int j = min(j0 + g0 + jj, Lr - 1);
+            const bool live = full || j0 + g0 + jj < Lr;
+            const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
+            int k;
+            gather_offsets<DT, GT, INTERP>((unsigned) a * Ku, Ku, (tt + tr[g0 + jj]) * fs, live,
+                                           o0[jj], o1[jj], f[jj], k);
+            if (STATUS) oob |= live && (unsigned) k >= Ku;
+          }
+#pragma unroll
+          for (int jj = 0; jj < kG; jj++) {
+            y0[jj] = bload<DT>(rY, o0[jj]);
+            if (INTERP != DREAM_HIP_NEAREST) y1[jj] = bload<DT>(rY, o1[jj]);
+          }
+#pragma unroll
+          for (int jj = 0; jj < kG; jj++) {
+            const DT v = INTERP == DREAM_HIP_NEAREST ? y0[jj] : y0[jj] + f[jj] * (y1[jj] - y0[jj]);
+            part += APOD ? ar[min(j0 + g0 + jj, Lr - 1)] * v : v;
+          }
+        }
         acc += APOD ? wi * part : part;
       }
     }
@@ -579,6 +569,7 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj)
   const GT fs = (GT) A.fs_khz, inv_cp = (GT) A.inv_cp, dl = (GT) A.delay_ms;
   const GT tmax = (GT) A.leg_max;
   const GT bnd = (GT) (0x1p-40 * (fabs(2.0 * (double) fs * (double) dl) + 1.0));
+  const GT fs2 = fs + fs;
   const DT *g = (const DT *) A.g + frame * A.im_stride;
   const DT *at = (const DT *) A.apod_t;
   const DT *ar = (const DT *) A.apod_r;
@@ -728,7 +719,7 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj)
         GT tleg;
         u[q] = saft_pos<GT, kSaftFast<DT, GT, INTERP>>(Gt[a] - xo[q], Gt[a + Lt] - yo[q],
                                                       Gt[a + 2 * Lt] - zo[q], inv_cp, dl, tmax,
-                                                      fs, bnd, tleg);
+                                                      fs, fs2, bnd, tleg);
         tsum[q] = tleg + tleg;
       } else {
         const int i = min(i0 + ii, Lt - 1);

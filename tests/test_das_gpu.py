@@ -84,7 +84,7 @@ def test_ramp_indices_exact_all_precisions(gpu, oracle, method):
 @pytest.mark.parametrize("delay", [0.0, 0.37])
 def test_saft_positions_on_sample_boundaries(gpu, oracle, delay):
     """Pixels straight below one element whose sample position 2 (z/cp +
-    delay) fs lies within 1e-13 .. 1e-7 of an integer, on both sides: the
+    delay) fs lies within 1e-13 .. 1e-4 of an integer, on both sides: the
     kernel's fast square root is certified there or replaced by the IEEE one,
     so the FP64 indices (ramp data) must come out bit for bit, for FP64 data
     and for FP32 data with FP64 delays."""
@@ -96,6 +96,7 @@ def test_saft_positions_on_sample_boundaries(gpu, oracle, delay):
     z = []
     for n in np.arange(200, 3800, 240):
         z0 = (n / (2.0 * fs) - delay / 1000.0) * cp
+        z += [z0 * (1.0 + m * 2.0 ** -30) for m in range(-64, 65)]
         z += [z0 * (1.0 + m * 2.0 ** -40) for m in range(-64, 65)]
         z += [z0 * (1.0 + m * 2.0 ** -52) for m in range(-20, 21)]
     z = np.array(z)
@@ -109,7 +110,7 @@ def test_saft_positions_on_sample_boundaries(gpu, oracle, delay):
     Im = plan.forward(Yt, dt, delay, cp).cpu().numpy().astype(np.float64)
     assert np.array_equal(Im, ref)
     # Linear interpolation of a ramp returns the position itself, here within
-    # 1e-7 of an integer (FP32 output: within 1e-3).
+    # 1e-4 of an integer (FP32 output: within 1e-3).
     lin = plan.forward(Yt, dt, delay, cp, interp="linear").cpu().numpy().astype(np.float64)
     assert np.max(np.abs(lin - np.rint(lin))) <= 1e-3
     assert np.max(np.abs(lin - ref)) <= 1.0 + 1e-3
