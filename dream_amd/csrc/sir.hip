@@ -49,21 +49,31 @@ namespace {
 constexpr int kThreads = 256;
 constexpr double kBoundScale = 0x1p-36;
 
-// Sample index (as a double) and 1/r for squared distance s.  `lim` is
+// Sample index (as a double) of the sub-element at offset (rx, ry, rz), and
+// h = 1/(2 r) for its amplitude; ryy = ry*ry, rzz = rz*rz, ryz = ryy + rzz.
+// The estimate takes s = rx*rx + ryz in one FMA (within 2^-50 of the
+// reference's (rx*rx + ry*ry) + rz*rz, far inside the certification
+// margin), r and 1/(2r) from the hardware rsqrt seed and one Goldschmidt
+// step; the exact path recomputes s in the reference's order.  `lim` is
 // 0.5 - B (see the file comment): |u - rint(u)| <= lim certifies rint(u).
-template <int STEPS>
-__device__ __forceinline__ double sir_index(double s, double c1, double c2, double lim,
-                                            double cp, double d, double dt, double &rinv)
+__device__ __forceinline__ double sir_index(double rx, double ryy, double rzz, double ryz,
+                                            double c1, double c2, double lim, double cp, double d,
+                                            double dt, double &h)
 {
-  double r;
-  sqrt_rsqrt<STEPS>(s, r, rinv);
-  const double u = __builtin_fma(r, c1, -c2);
+  const double s = __builtin_fma(rx, rx, ryz);
+  const double y = __builtin_amdgcn_rsq(s);
+  double g = s * y;
+  h = 0.5 * y;
+  const double e = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  const double u = __builtin_fma(g, c1, -c2);
   double ru = __builtin_rint(u);
   if (!(__builtin_fabs(u - ru) <= lim)) {        // also s = 0, NaN, inf
-    const double re = __builtin_sqrt(s);
+    const double re = __builtin_sqrt((rx * rx + ryy) + rzz);
     const double t = (re * 1000.0) / cp;
     ru = __builtin_rint((t - d) / dt);
-    rinv = 1.0 / re;
+    h = 0.5 / re;
   }
   return ru;
 }
@@ -143,11 +153,9 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
   int e = q / ny;
   int j = q - e * ny;
   int row = (int) (((int64_t) j * P.row_stride) % ny);
-  // One Goldschmidt step: gfx950's FP64 rsqrt seed is accurate to about
-  // 2^-24, which one step takes to about 2^-47 -- enough for the index
-  // certification and for FP64 amplitudes.
-  constexpr int kSteps = 1;
-
+  // One Goldschmidt step (in sir_index): gfx950's FP64 rsqrt seed is
+  // accurate to about 2^-24, which one step takes to about 2^-47 -- enough
+  // for the index certification and for FP64 amplitudes.
   for (; q < q1; q += kThreads) {
     const double xe = xo - el[e];
     const double ye = yo - el[L + e];
@@ -161,14 +169,13 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
     const double rzz = rz * rz;
     const double c2 = d * P.inv_dt;
     const double lim = 0.5 - kBoundScale * (ntd + 2.0 + __builtin_fabs(c2));
-    const double amp = el[5 * L + e] * P.amp;
-    const bool last = (e == L - 1);
+    const double ryz = ryy + rzz;
+    // ai = amp / r = (2 amp) h, added into the run by one FMA.
+    const double amp2 = 2.0 * (el[5 * L + e] * P.amp);
+    const unsigned oob_row = oob;
     auto contribution = [&](double x) {
-      const double rx = xe - x;
-      const double s = (rx * rx + ryy) + rzz;
-      double rinv;
-      const double ru = sir_index<kSteps>(s, P.c1, c2, lim, P.cp, d, P.dt, rinv);
-      const double ai = amp * rinv;
+      double h;
+      const double ru = sir_index(xe - x, ryy, rzz, ryz, P.c1, c2, lim, P.cp, d, P.dt, h);
       if (ru >= 0.0 && ru < ntd) {
         const int bin = (int) ru;
         if (bin != cur) {
@@ -176,13 +183,11 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
             if (use_lds) lds_add(hist + cur, acc); else global_add(out + cur, (T) acc);
           }
           cur = bin;
-          acc = ai;
-        } else {
-          acc += ai;
+          acc = 0.0;
         }
+        acc = __builtin_fma(amp2, h, acc);
       } else {
         oob++;
-        oob_last |= last;
       }
     };
     if (P.shape == SIR_CIRC) {
@@ -206,6 +211,7 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
       }
       for (; i < nx; i++) contribution(xs[i]);
     }
+    oob_last |= (e == L - 1) && oob != oob_row;
     e += P.step_e;
     j += P.step_j;
     if (j >= ny) { j -= ny; e++; }
@@ -293,15 +299,16 @@ __global__ void rect_indices_kernel(double xo, double yo, double zo, const doubl
   const int row = (int) (q / nx), col = (int) (q - (int64_t) row * nx);
   const double ry = yo - ys[row];
   const double rx = xo - xs[col];
-  const double s = (rx * rx + ry * ry) + zo * zo;
   const double c1 = 1000.0 / (cp * dt);
   const double c2 = delay * (1.0 / dt);
   // Every index is wanted here, so the bound covers this contribution's |u|.
   double r, rinv;
-  sqrt_rsqrt<1>(s, r, rinv);
+  sqrt_rsqrt<1>((rx * rx + ry * ry) + zo * zo, r, rinv);
   const double umax = __builtin_fabs(__builtin_fma(r, c1, -c2));
   const double lim = 0.5 - kBoundScale * (umax + 2.0 + __builtin_fabs(c2));
-  it_out[q] = (int64_t) sir_index<1>(s, c1, c2, lim, cp, delay, dt, rinv);
+  double h;
+  it_out[q] = (int64_t) sir_index(rx, ry * ry, zo * zo, ry * ry + zo * zo, c1, c2, lim, cp, delay,
+                                  dt, h);
 }
 
 int gcd_int(int a, int b)
