@@ -539,7 +539,7 @@ __device__ __forceinline__ int window_start(double umin)
 
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
 __global__ void __launch_bounds__(kThreads)
-das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj)
+das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj, int subs)
 {
   constexpr int NA = TYPE == DAS_SAFT ? kSaftGroup : kGI * kGJ;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -578,31 +578,24 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj)
     y_lo = Gr[Lr]; y_hi = Gr[2 * Lr - 1]; x_lo = Gt[0]; x_hi = Gt[Lt - 1];
   }
 
-  // The lane's pixels.
-  GT xo[kRPU], yo[kRPU], zo[kRPU];
-  DT gp[kRPU];
-  bool live[kRPU];
-  const int64_t base = region * kRegion;
-  GT bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+  // The work-group covers `subs` consecutive regions, one after the other,
+  // with one set of windows (flushed once) spanning all of them.
   // Lanes take their 64-pixel block transposed (lane l -> position
   // 8 (l mod 8) + l / 8): the visiting order runs along x inside a block,
   // and consecutive lanes a row apart scatter into different LDS bins.
   const int lane_pos = (int) ((threadIdx.x & ~63u) | ((threadIdx.x & 7u) << 3) | ((threadIdx.x >> 3) & 7u));
-#pragma unroll
-  for (int q = 0; q < kRPU; q++) {
-    const int64_t idx = base + (int64_t) q * kThreads + lane_pos;
-    live[q] = idx < A.No;
-    const int64_t p = pixel_of(A, perm, live[q] ? idx : base);
-    xo[q] = Ro[p]; yo[q] = Ro[p + A.No]; zo[q] = Ro[p + 2 * A.No];
-    gp[q] = live[q] ? g[p] : DT(0);
-    if (live[q]) {
-      bmin[0] = fmin(bmin[0], xo[q]); bmax[0] = fmax(bmax[0], xo[q]);
-      bmin[1] = fmin(bmin[1], yo[q]); bmax[1] = fmax(bmax[1], yo[q]);
-      bmin[2] = fmin(bmin[2], zo[q]); bmax[2] = fmax(bmax[2], zo[q]);
-    }
-  }
-  // Bounding box of the region (only needed for the LDS windows).
+  const int64_t base0 = region * (int64_t) subs * kRegion;
+  // Bounding box of the work-group's pixels (only needed for the LDS windows).
   if (want_gy) {
+    GT bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t idx = base0 + lane_pos; idx < base0 + (int64_t) subs * kRegion && idx < A.No;
+         idx += kThreads) {
+      const int64_t p = pixel_of(A, perm, idx);
+      const GT x = Ro[p], y = Ro[p + A.No], z = Ro[p + 2 * A.No];
+      bmin[0] = fmin(bmin[0], x); bmax[0] = fmax(bmax[0], x);
+      bmin[1] = fmin(bmin[1], y); bmax[1] = fmax(bmax[1], y);
+      bmin[2] = fmin(bmin[2], z); bmax[2] = fmax(bmax[2], z);
+    }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       GT mn = bmin[c], mx = bmax[c];
@@ -661,28 +654,6 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj)
   }
   __syncthreads();
 
-  // Two-leg: receive legs of the group for the lane's pixels.
-  GT tr[TYPE == DAS_SAFT ? 1 : kGJ][kRPU];
-  GT gx_r[kRPU], gy_c[kRPU];
-  if (TYPE != DAS_SAFT) {
-#pragma unroll
-    for (int q = 0; q < kRPU; q++) {
-      gx_r[q] = GT(0); gy_c[q] = GT(0);
-      if (TYPE != DAS_TFM) {
-        gy_c[q] = stripe_gap<GT>(yo[q], y_lo, y_hi);
-        gx_r[q] = stripe_gap<GT>(xo[q], x_lo, x_hi);
-      }
-#pragma unroll
-      for (int jj = 0; jj < kGJ; jj++) {
-        const int j = min(j0 + jj, Lr - 1);
-        if (TYPE == DAS_TFM)
-          tr[jj][q] = leg3<GT>(Gr[j] - xo[q], Gr[j + Lr] - yo[q], Gr[j + 2 * Lr] - zo[q]) * inv_cp;
-        else
-          tr[jj][q] = leg3<GT>(gx_r[q], Gr[j + Lr] - yo[q], Gr[j + 2 * Lr] - zo[q]) * inv_cp;
-      }
-    }
-  }
-
   const __amdgpu_buffer_rsrc_t ry =
       frame_rsrc((const DT *) A.Y + frame * A.y_stride, A.y_stride * (int64_t) sizeof(DT));
   DT *gy_frame = want_gy ? (DT *) A.gY + frame * A.y_stride : nullptr;
@@ -691,78 +662,117 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj)
 #pragma unroll
   for (int s = 0; s < NA; s++) zsum[s] = DT(0);
 
-#pragma unroll
-  for (int s = 0; s < NA; s++) {
-    int a, ii = 0, jj = 0;
-    bool ok;
-    DT w;
-    if (TYPE == DAS_SAFT) {
-      const int n = n0 + s;
-      ok = n < Lt;
-      a = min(n, Lt - 1);
-      w = APOD ? at[a] : DT(1);
-    } else {
-      ii = s / kGJ; jj = s % kGJ;
-      const int i = i0 + ii, j = j0 + jj;
-      ok = i < Lt && j < Lr;
-      const int ic = min(i, Lt - 1), jc = min(j, Lr - 1);
-      a = (TYPE == DAS_RCA_ROWTX) ? jc * Lt + ic : ic * Lr + jc;
-      w = APOD ? at[ic] * ar[jc] : DT(1);
-    }
-    if (!ok) continue;                           // uniform
-    const unsigned abase = (unsigned) a * (unsigned) K;
-    const int w0 = want_gy ? s_w0[s] : 0;
-    GT u[kRPU], tsum[kRPU];
+  for (int sub = 0; sub < subs; sub++) {
+    // The lane's pixels in this region.
+    GT xo[kRPU], yo[kRPU], zo[kRPU];
+    DT gp[kRPU];
+    bool live[kRPU];
+    const int64_t base = base0 + (int64_t) sub * kRegion;
 #pragma unroll
     for (int q = 0; q < kRPU; q++) {
-      if (TYPE == DAS_SAFT) {
-        GT tleg;
-        u[q] = saft_pos<GT, kSaftFast<DT, GT, INTERP>>(Gt[a] - xo[q], Gt[a + Lt] - yo[q],
-                                                      Gt[a + 2 * Lt] - zo[q], inv_cp, dl, tmax,
-                                                      fs, fs2, bnd, tleg);
-        tsum[q] = tleg + tleg;
-      } else {
-        const int i = min(i0 + ii, Lt - 1);
-        GT tt;
-        if (TYPE == DAS_TFM)
-          tt = leg3<GT>(Gt[i] - xo[q], Gt[i + Lt] - yo[q], Gt[i + 2 * Lt] - zo[q]) * inv_cp;
-        else
-          tt = leg3<GT>(Gt[i] - xo[q], gy_c[q], Gt[i + 2 * Lt] - zo[q]) * inv_cp;
-        tsum[q] = tt + tr[jj][q];
-        tt += dl;
-        u[q] = (clamp_leg<GT>(tt, tmax) + clamp_leg<GT>(tr[jj][q], tmax)) * fs;
+      const int64_t idx = base + (int64_t) q * kThreads + lane_pos;
+      live[q] = idx < A.No;
+      const int64_t p = pixel_of(A, perm, live[q] ? idx : 0);
+      xo[q] = Ro[p]; yo[q] = Ro[p + A.No]; zo[q] = Ro[p + 2 * A.No];
+      gp[q] = live[q] ? g[p] : DT(0);
+    }
+
+    // Two-leg: receive legs of the group for the lane's pixels.
+    GT tr[TYPE == DAS_SAFT ? 1 : kGJ][kRPU];
+    GT gx_r[kRPU], gy_c[kRPU];
+    if (TYPE != DAS_SAFT) {
+#pragma unroll
+      for (int q = 0; q < kRPU; q++) {
+        gx_r[q] = GT(0); gy_c[q] = GT(0);
+        if (TYPE != DAS_TFM) {
+          gy_c[q] = stripe_gap<GT>(yo[q], y_lo, y_hi);
+          gx_r[q] = stripe_gap<GT>(xo[q], x_lo, x_hi);
+        }
+#pragma unroll
+        for (int jj = 0; jj < kGJ; jj++) {
+          const int j = min(j0 + jj, Lr - 1);
+          if (TYPE == DAS_TFM)
+            tr[jj][q] = leg3<GT>(Gr[j] - xo[q], Gr[j + Lr] - yo[q], Gr[j + 2 * Lr] - zo[q]) * inv_cp;
+          else
+            tr[jj][q] = leg3<GT>(gx_r[q], Gr[j + Lr] - yo[q], Gr[j + 2 * Lr] - zo[q]) * inv_cp;
+        }
       }
     }
-    DT y0[kRPU], y1[kRPU], f[kRPU];
-    int k[kRPU];
+
 #pragma unroll
-    for (int q = 0; q < kRPU; q++)
-      fetch_taps<DT, GT, INTERP>(ry, abase, K, u[q], live[q], y0[q], y1[q], f[q], k[q]);
+    for (int s = 0; s < NA; s++) {
+      int a, ii = 0, jj = 0;
+      bool ok;
+      DT w;
+      if (TYPE == DAS_SAFT) {
+        const int n = n0 + s;
+        ok = n < Lt;
+        a = min(n, Lt - 1);
+        w = APOD ? at[a] : DT(1);
+      } else {
+        ii = s / kGJ; jj = s % kGJ;
+        const int i = i0 + ii, j = j0 + jj;
+        ok = i < Lt && j < Lr;
+        const int ic = min(i, Lt - 1), jc = min(j, Lr - 1);
+        a = (TYPE == DAS_RCA_ROWTX) ? jc * Lt + ic : ic * Lr + jc;
+        w = APOD ? at[ic] * ar[jc] : DT(1);
+      }
+      if (!ok) continue;                           // uniform
+      const unsigned abase = (unsigned) a * (unsigned) K;
+      const int w0 = want_gy ? s_w0[s] : 0;
+      GT u[kRPU], tsum[kRPU];
 #pragma unroll
-    for (int q = 0; q < kRPU; q++) {
-      const DT slope = INTERP == DREAM_HIP_LINEAR ? y1[q] - y0[q] : DT(0);
-      zsum[s] += gp[q] * (y0[q] + f[q] * slope);
-      if (INTERP == DREAM_HIP_LINEAR) zc += gp[q] * w * slope * (DT) tsum[q];
-      if (want_gy && k[q] > -2) {
-        const DT gw = gp[q] * w;
-        if (INTERP == DREAM_HIP_NEAREST) {
-          const unsigned l = (unsigned) (k[q] - w0);
-          if (l < (unsigned) kWin) lds_atomic_add(&win[s * kWin + l], (double) gw);
-          else unsafeAtomicAdd(&gy_frame[abase + k[q]], gw);
+      for (int q = 0; q < kRPU; q++) {
+        if (TYPE == DAS_SAFT) {
+          GT tleg;
+          u[q] = saft_pos<GT, kSaftFast<DT, GT, INTERP>>(Gt[a] - xo[q], Gt[a + Lt] - yo[q],
+                                                        Gt[a + 2 * Lt] - zo[q], inv_cp, dl, tmax,
+                                                        fs, fs2, bnd, tleg);
+          tsum[q] = tleg + tleg;
         } else {
-          const unsigned l = (unsigned) (k[q] - w0);
-          const DT v0 = gw * (DT(1) - f[q]), v1 = gw * f[q];
-          if (l + 1u < (unsigned) kWin) {         // both taps in the window
-            lds_atomic_add(&win[s * kWin + l], (double) v0);
-            lds_atomic_add(&win[s * kWin + l + 1], (double) v1);
+          const int i = min(i0 + ii, Lt - 1);
+          GT tt;
+          if (TYPE == DAS_TFM)
+            tt = leg3<GT>(Gt[i] - xo[q], Gt[i + Lt] - yo[q], Gt[i + 2 * Lt] - zo[q]) * inv_cp;
+          else
+            tt = leg3<GT>(Gt[i] - xo[q], gy_c[q], Gt[i + 2 * Lt] - zo[q]) * inv_cp;
+          tsum[q] = tt + tr[jj][q];
+          tt += dl;
+          u[q] = (clamp_leg<GT>(tt, tmax) + clamp_leg<GT>(tr[jj][q], tmax)) * fs;
+        }
+      }
+      DT y0[kRPU], y1[kRPU], f[kRPU];
+      int k[kRPU];
+#pragma unroll
+      for (int q = 0; q < kRPU; q++)
+        fetch_taps<DT, GT, INTERP>(ry, abase, K, u[q], live[q], y0[q], y1[q], f[q], k[q]);
+#pragma unroll
+      for (int q = 0; q < kRPU; q++) {
+        const DT slope = INTERP == DREAM_HIP_LINEAR ? y1[q] - y0[q] : DT(0);
+        zsum[s] += gp[q] * (y0[q] + f[q] * slope);
+        if (INTERP == DREAM_HIP_LINEAR) zc += gp[q] * w * slope * (DT) tsum[q];
+        if (want_gy && k[q] > -2) {
+          const DT gw = gp[q] * w;
+          if (INTERP == DREAM_HIP_NEAREST) {
+            const unsigned l = (unsigned) (k[q] - w0);
+            if (l < (unsigned) kWin) lds_atomic_add(&win[s * kWin + l], (double) gw);
+            else unsafeAtomicAdd(&gy_frame[abase + k[q]], gw);
           } else {
-            if (k[q] >= 0) unsafeAtomicAdd(&gy_frame[abase + k[q]], v0);
-            if (k[q] + 1 < K) unsafeAtomicAdd(&gy_frame[abase + k[q] + 1], v1);
+            const unsigned l = (unsigned) (k[q] - w0);
+            const DT v0 = gw * (DT(1) - f[q]), v1 = gw * f[q];
+            if (l + 1u < (unsigned) kWin) {         // both taps in the window
+              lds_atomic_add(&win[s * kWin + l], (double) v0);
+              lds_atomic_add(&win[s * kWin + l + 1], (double) v1);
+            } else {
+              if (k[q] >= 0) unsafeAtomicAdd(&gy_frame[abase + k[q]], v0);
+              if (k[q] + 1 < K) unsafeAtomicAdd(&gy_frame[abase + k[q] + 1], v1);
+            }
           }
         }
       }
     }
-  }
+
+  }  // sub-regions
 
   // Per-A-scan sums and the cp sum: wave, then work-group, then one atomic.
 #pragma unroll
@@ -1109,7 +1119,12 @@ void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t
   const int64_t n_ascans = (TYPE == DAS_SAFT) ? a.Lt : (int64_t) a.Lt * a.Lr;
   if (TYPE == DAS_SAFT || want_gy) {
     // Region-major: (pixel region, A-scan group, frame) work-groups.
-    const int64_t regions = (a.No + kRegion - 1) / kRegion;
+    // A work-group takes `subs` consecutive regions (one window flush for
+    // all of them).  Candidates 1, 2, 4 while the grid keeps at least four
+    // work-groups per CU; the autotuner picks (the windows of a larger
+    // group cover a wider delay range, which coarse pixel grids can
+    // overflow into direct gY atomics), the first is the default.
+    const int64_t regions1 = (a.No + kRegion - 1) / kRegion;
     int n_gj = 1;
     int64_t groups;
     if (TYPE == DAS_SAFT) {
@@ -1118,12 +1133,42 @@ void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t
       n_gj = (a.Lr + kGJ - 1) / kGJ;
       groups = (int64_t) ((a.Lt + kGI - 1) / kGI) * n_gj;
     }
+    std::vector<int> subs_cand{1};
+    for (int sb = 2; sb <= 4; sb *= 2)
+      if ((regions1 + sb - 1) / sb * groups * a.n_frames >= 4 * (int64_t) cu_count())
+        subs_cand.insert(subs_cand.begin(), sb);
     const int na = TYPE == DAS_SAFT ? kSaftGroup : kGI * kGJ;
     const size_t lds = want_gy ? (size_t) na * kWin * sizeof(double) : 0;
-    if (want_gy) zero_async(a.gY, (size_t) (a.n_frames * a.y_stride) * sizeof(DT), s);
-    dim3 grid((unsigned) regions, (unsigned) groups, (unsigned) a.n_frames);
-    hipLaunchKernelGGL((das_adj_region_kernel<DT, GT, TYPE, INTERP, APOD>), grid,
-                       dim3(kThreads), lds, s, a, perm, n_gj);
+    const int64_t n_z = TYPE == DAS_SAFT ? a.Lt : (int64_t) a.Lt * a.Lr;
+    auto run = [&](int subs, bool clear_z, hipStream_t st) {
+      if (clear_z) {
+        zero_async(a.z_t, (size_t) n_z * sizeof(double), st);
+        zero_async(a.z_cp, sizeof(double), st);
+      }
+      if (want_gy) zero_async(a.gY, (size_t) (a.n_frames * a.y_stride) * sizeof(DT), st);
+      const int64_t regions = (regions1 + subs - 1) / subs;
+      dim3 grid((unsigned) regions, (unsigned) groups, (unsigned) a.n_frames);
+      hipLaunchKernelGGL((das_adj_region_kernel<DT, GT, TYPE, INTERP, APOD>), grid,
+                         dim3(kThreads), lds, st, a, perm, n_gj, subs);
+    };
+    int pick = 0;
+    if (subs_cand.size() > 1) {
+      char key[192];
+      std::snprintf(key, sizeof key, "das_adj|%d|%d|%d|%d|%d|%d|%lld|%d|%d|%lld|%lld", TYPE,
+                    (int) sizeof(DT), (int) sizeof(GT), INTERP, (int) APOD, (int) want_gy,
+                    (long long) a.No, a.Lt, a.Lr, (long long) a.K, (long long) a.n_frames);
+      const int before = autotune_cache_size();
+      const int t = autotune(key, (int) subs_cand.size(),
+                             [&](int i, hipStream_t st) { run(subs_cand[(size_t) i], true, st); },
+                             s);
+      if (t >= 0) pick = t;
+      // Candidates ran here: start the accumulators again.
+      if (autotune_cache_size() != before) {
+        zero_async(a.z_t, (size_t) n_z * sizeof(double), s);
+        zero_async(a.z_cp, sizeof(double), s);
+      }
+    }
+    run(subs_cand[(size_t) pick], false, s);
   } else {
     const int64_t tiles = ((a.No + kThreads - 1) / kThreads) * a.n_frames;
     const int grid = (int) std::min<int64_t>(tiles, (int64_t) cu_count() * 4);

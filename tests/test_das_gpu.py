@@ -320,6 +320,41 @@ def test_uneven_batch_graph_replays_and_oracle(gpu, oracle):
         assert np.max(np.abs(o.cpu().numpy() - ref)) <= 1e-12 * max(np.abs(ref).max(), 1.0)
 
 
+@pytest.mark.parametrize("pitch,K", [(0.025, 1024), (0.1, 4096)])
+def test_saft_adjoint_multi_region_work_groups(gpu, oracle, pitch, K):
+    """A SAFT image large enough for adjoint work-groups of 2 and 4 regions
+    (one window flush per work-group).  The coarse grid makes the windows of
+    4-region groups overflow into direct gY atomics.  Gradients against the
+    FP64 oracle with tuning off (the default, 4 regions) and on (the tuned
+    pick, after the candidates ran on the same accumulators)."""
+    from dream_amd import _lib
+    from dream_amd.torch_ops import DasPlan
+    rng = np.random.default_rng(41)
+    L, n = 128, 512
+    x = np.linspace(-6.35, 6.35, L)
+    Gt = np.c_[x, np.zeros(L), np.zeros(L)]
+    X, Z = np.meshgrid(pitch * (np.arange(n) - n / 2), 2.0 + pitch * np.arange(n))
+    Ro = np.c_[X.ravel("F"), np.zeros(n * n), Z.ravel("F")]
+    Y = rng.standard_normal((K, L))
+    at = rng.uniform(0.5, 1.5, L)
+    g = rng.standard_normal(n * n)
+    gY, gat, _, gcp = oracle.das_ext_adjoint(Y, Gt, None, Ro, 0.02, 0.1, 1500.0, g, "saft",
+                                             "linear", at, None, nthreads=8)
+    plan = DasPlan("saft", K, Ro, Gt, None, precision="f64")
+    Yt = torch.tensor(Y.T.copy(), device="cuda")
+    att = torch.tensor(at, device="cuda")
+    gt = torch.tensor(g, device="cuda")
+    try:
+        for tune in (0, 1, 1):
+            _lib.lib().dream_hip_set_autotune(tune)
+            rY, rat, _, rcp = plan.adjoint(Yt, gt, 0.02, 0.1, 1500.0, att, None, "linear")
+            assert rel_l2(rY.cpu().numpy().T, gY) <= 1e-10
+            assert rel_l2(rat.cpu().numpy(), gat) <= 1e-10
+            assert abs(float(rcp) - gcp) <= 1e-9 * abs(gcp)
+    finally:
+        _lib.lib().dream_hip_set_autotune(1)
+
+
 def test_autotuned_shapes_give_the_same_image(gpu):
     """The forward launch shape picked by the autotuner (waves per work-group,
     split) changes only the summation order: ramp data give identical images
