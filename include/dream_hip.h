@@ -74,9 +74,10 @@ const char *dream_hip_version(void);
 int dream_hip_device_count(void);
 
 /* Launch-shape autotuning (on by default; DREAM_HIP_AUTOTUNE=0 turns it
- * off at start-up).  When on, the first launch of a DAS forward, DAS batch
- * or SIR batch shape class times the candidate launch shapes (work-group
- * size, split, persistent grid) on the caller's stream and keeps the
+ * off at start-up).  When on, the first launch of a DAS forward, DAS
+ * region adjoint, DAS batch or SIR batch shape class times the candidate
+ * launch shapes (work-group size, split, regions per work-group, persistent
+ * grid) on the caller's stream and keeps the
  * fastest, cached per process by (device architecture, shape class) and,
  * when DREAM_HIP_TUNE_CACHE names a file, in that file.  Results are the
  * same for every shape up to the summation order.  No counterpart in the
