@@ -142,7 +142,10 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
 
   unsigned oob = 0;
   bool oob_last = false;
-  int cur = -1;
+  // The run's bin.  With the LDS histogram it starts at the spare bin nt
+  // past the column, so every flush is unconditional (the first adds 0
+  // there); in global mode -1 marks "no run yet".
+  int cur = use_lds ? (int) nt : -1;
   double acc = 0.0;
   gdouble *el = (gdouble *) P.el;
   gdouble *ys = (gdouble *) P.ys;
@@ -179,9 +182,8 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
       if (ru >= 0.0 && ru < ntd) {
         const int bin = (int) ru;
         if (bin != cur) {
-          if (cur >= 0) {
-            if (use_lds) lds_add(hist + cur, acc); else global_add(out + cur, (T) acc);
-          }
+          if (use_lds) lds_add(hist + cur, acc);
+          else if (cur >= 0) global_add(out + cur, (T) acc);
           cur = bin;
           acc = 0.0;
         }
@@ -218,9 +220,8 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
     row += P.step_row;
     if (row >= ny) row -= ny;
   }
-  if (cur >= 0) {
-    if (use_lds) lds_add(hist + cur, acc); else global_add(out + cur, (T) acc);
-  }
+  if (use_lds) lds_add(hist + cur, acc);
+  else if (cur >= 0) global_add(out + cur, (T) acc);
 
   // Status: wave-reduced, one atomic per wave.
   const unsigned wsum = wave_sum(oob);
@@ -653,7 +654,7 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
       chunks = std::min<int64_t>(chunks, std::max<int64_t>(Q / kThreads, 1));
     }
     d.chunks = (int32_t) std::max<int64_t>(chunks, 1);
-    const bool fits = (size_t) p.nt * esz <= lds_cap;
+    const bool fits = (size_t) (p.nt + 1) * esz <= lds_cap;     // + the spare bin
     d.mode = !fits ? SIR_GLOBAL : (d.chunks > 1 ? SIR_LDS_ATOMIC : SIR_LDS_STORE);
     if (fits) max_nt_lds = std::max(max_nt_lds, p.nt);
     cost[i] = (double) Q * (double) nx / d.chunks;
@@ -707,7 +708,7 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
     }
   }
   n_items_ = item;
-  lds_bytes_ = (size_t) max_nt_lds * esz;
+  lds_bytes_ = max_nt_lds ? (size_t) (max_nt_lds + 1) * esz : 0;
   if (!tables.empty())
     DREAM_HIP_CHECK(hipMemcpy(d_tables_, tables.data(), tables.size() * sizeof(double),
                               hipMemcpyHostToDevice));
