@@ -15,7 +15,9 @@ with the data already resident in HBM.  value = pixel-channel sums per second
   extra         the other measured paths (FP64 / reference-float TFM, the C2
                 image through the host-buffer entry point with its PCIe
                 copies, a 256 MiB device copy, C1 SIR, C3 batched SIR, C4
-                SAFT forward/backward, C5 RCA frames).
+                SAFT forward/backward, C5 RCA frames) and, beside the
+                headline's cpu_baseline, the oracle's SIR timed on the host
+                cores (all of C1, a bounded sample of C3).
 Only one GPU is used; --gpus N > 1 exits with a message.
 """
 import argparse
@@ -433,6 +435,55 @@ def cpu_baseline_c2(w, n_pix, n_ch):
                       f"host, {threads} threads, median of 3"}
 
 
+def cpu_baseline_c1():
+    """C1 SIR on the host cores: the oracle's FP64 restatement of
+    dream_arr_rect (the reference's own SIR source needs FFTW, which this
+    image lacks), threaded like the reference, median of 3."""
+    import numpy as np
+
+    import workloads
+    from oracle import oracle as O
+    threads = cpu_threads()
+    w = workloads.c1()
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        O.dream_arr_rect(w["Ro"], w["a"], w["b"], w["dx"], w["dy"], w["dt"], w["nt"],
+                         w["delay"], w["v"], w["cp"], w["G"], err_level="stop", nthreads=threads)
+        times.append(time.perf_counter() - t)
+    t = sorted(times)[1]
+    n_oe = w["Ro"].shape[0] * w["G"].shape[0]
+    return dict(ms=round(t * 1e3, 3), obs_elem_per_s=n_oe / t, cores=threads, kind="port",
+                sample="full C1 problem (64 points x 16 elements x 3600 sub-elements), FP64, "
+                       f"{threads} threads, median of 3")
+
+
+def cpu_baseline_c3_sample(n_points=512):
+    """A bounded sample of C3 on the host cores: the first problem's first
+    n_points observation points with all its elements, the oracle's FP64
+    restatement, threaded; reported as sub-element contributions per second
+    (the C3 line's unit), median of 3."""
+    import workloads
+    from oracle import oracle as O
+    threads = cpu_threads()
+    p = dict(workloads.c3()[0])
+    p["Ro"] = p["Ro"][:n_points]
+    p["delay"] = p["delay"][:n_points]
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        O.dream_arr_rect(p["Ro"], p["a"], p["b"], p["dx"], p["dy"], p["dt"], p["nt"], p["delay"],
+                         p["v"], p["cp"], p["G"], foc_met=p["foc_met"], focal=p["focal"],
+                         apod_met=p["apod_met"], err_level="ignore", nthreads=threads)
+        times.append(time.perf_counter() - t)
+    t = sorted(times)[1]
+    contrib = _sir_contributions(p)
+    return dict(ms=round(t * 1e3, 3), contributions_per_s=contrib / t, contributions=contrib,
+                cores=threads, kind="port",
+                sample=f"C3 problem 0, first {p['Ro'].shape[0]} points x {p['G'].shape[0]} "
+                       f"elements, FP64, {threads} threads, median of 3")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -480,6 +531,9 @@ def main():
         extra.update(bench_batches(args.steps, args.warmup))
         extra.update(bench_conv(args.steps, args.warmup))
         extra.update(bench_das_diff(args.steps, args.warmup))
+        if not args.no_cpu:
+            extra["c1_sir_cpu_baseline"] = cpu_baseline_c1()
+            extra["c3_sir_cpu_sample"] = cpu_baseline_c3_sample()
     line = {
         "metric": METRIC,
         "value": res["value"],
