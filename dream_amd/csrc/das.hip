@@ -85,7 +85,7 @@ __device__ __forceinline__ double wave_sum_d(double v)
   return v;
 }
 
-__device__ __forceinline__ int64_t pixel_of(const DasArgs &A, const int32_t *perm, int64_t idx)
+__device__ __forceinline__ int64_t pixel_of(const int32_t *perm, int64_t idx)
 {
   return perm ? (int64_t) perm[idx] : idx;
 }
@@ -218,7 +218,7 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
                                            bool &oob)
 {
   const bool valid = idx < A.No;
-  const int64_t p = valid ? pixel_of(A, perm, idx) : 0;
+  const int64_t p = valid ? pixel_of(perm, idx) : 0;
   const GT *Ro = (const GT *) A.Ro;
   const GT xo = Ro[p], yo = Ro[p + A.No], zo = Ro[p + 2 * A.No];
   const GT *gdel = (const GT *) A.delay;
@@ -381,7 +381,7 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
   const int sidx = blockIdx.y;
   const int64_t frame = blockIdx.z;
   const bool valid = idx < A.No;
-  const int64_t p = valid ? pixel_of(A, perm, idx) : 0;
+  const int64_t p = valid ? pixel_of(perm, idx) : 0;
   bool oob = false;
   const DT acc = fwd_tile_sum<DT, GT, TYPE, INTERP, APOD, STATUS>(
       A, perm, idx, frame, sidx * nwaves + wave, nwaves * split, oob);
@@ -449,7 +449,7 @@ das_batch_kernel(const DasArgs *__restrict__ args, const int64_t *__restrict__ i
     const int64_t idx = (item - item_begin[lo]) * 64 + lane;
     bool oob = false;
     const DT acc = fwd_tile_sum<DT, GT, TYPE, INTERP, APOD, false>(A, A.perm, idx, 0, 0, 1, oob);
-    if (idx < A.No) ((DT *) A.Im)[pixel_of(A, A.perm, idx)] = acc;
+    if (idx < A.No) ((DT *) A.Im)[pixel_of(A.perm, idx)] = acc;
   }
   if (lane == 0) {
     __threadfence();
@@ -590,7 +590,7 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj, int
     GT bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int64_t idx = base0 + lane_pos; idx < base0 + (int64_t) subs * kRegion && idx < A.No;
          idx += kThreads) {
-      const int64_t p = pixel_of(A, perm, idx);
+      const int64_t p = pixel_of(perm, idx);
       const GT x = Ro[p], y = Ro[p + A.No], z = Ro[p + 2 * A.No];
       bmin[0] = fmin(bmin[0], x); bmax[0] = fmax(bmax[0], x);
       bmin[1] = fmin(bmin[1], y); bmax[1] = fmax(bmax[1], y);
@@ -672,7 +672,7 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj, int
     for (int q = 0; q < kRPU; q++) {
       const int64_t idx = base + (int64_t) q * kThreads + lane_pos;
       live[q] = idx < A.No;
-      const int64_t p = pixel_of(A, perm, live[q] ? idx : 0);
+      const int64_t p = pixel_of(perm, live[q] ? idx : 0);
       xo[q] = Ro[p]; yo[q] = Ro[p + A.No]; zo[q] = Ro[p + 2 * A.No];
       gp[q] = live[q] ? g[p] : DT(0);
     }
@@ -855,7 +855,7 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
     const int64_t frame = tile / tiles_per_frame;
     const int64_t idx = (tile - frame * tiles_per_frame) * kThreads + threadIdx.x;
     const bool valid = idx < A.No;
-    const int64_t p = valid ? pixel_of(A, perm, idx) : 0;
+    const int64_t p = valid ? pixel_of(perm, idx) : 0;
     const GT xo = Ro[p], yo = Ro[p + A.No], zo = Ro[p + 2 * A.No];
     const DT gp = valid ? ((const DT *) A.g)[frame * A.im_stride + p] : DT(0);
     const __amdgpu_buffer_rsrc_t ry =
@@ -1116,7 +1116,6 @@ void fwd_dispatch_type(const DasArgs &a, int type, int interp, bool apod, hipStr
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
 void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t s)
 {
-  const int64_t n_ascans = (TYPE == DAS_SAFT) ? a.Lt : (int64_t) a.Lt * a.Lr;
   if (TYPE == DAS_SAFT || want_gy) {
     // Region-major: (pixel region, A-scan group, frame) work-groups.
     // A work-group takes `subs` consecutive regions (one window flush for
