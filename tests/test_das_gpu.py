@@ -355,6 +355,41 @@ def test_saft_adjoint_multi_region_work_groups(gpu, oracle, pitch, K):
         _lib.lib().dream_hip_set_autotune(1)
 
 
+@pytest.mark.parametrize("method", ["tfm", "rca_rowtx"])
+def test_two_leg_adjoint_multi_region_work_groups(gpu, oracle, method):
+    """Two-leg adjoints with the Y-gradient on an image large enough for
+    work-groups of several regions (tuning off: the largest; on: the pick),
+    against the FP64 oracle."""
+    from dream_amd import _lib
+    from dream_amd.torch_ops import DasPlan
+    rng = np.random.default_rng(42)
+    L, n, K = 32, 128, 1024
+    x = np.linspace(-3.1, 3.1, L)
+    Gt = np.c_[x, np.zeros(L), np.zeros(L)]
+    Gr = Gt.copy() if method == "tfm" else np.c_[np.zeros(L), x, np.zeros(L)]
+    X, Z = np.meshgrid(0.05 * (np.arange(n) - n / 2), 2.0 + 0.05 * np.arange(n))
+    Ro = np.c_[X.ravel("F"), np.zeros(n * n), Z.ravel("F")]
+    Y = rng.standard_normal((K, L * L))
+    at, ar = rng.uniform(0.5, 1.5, L), rng.uniform(0.5, 1.5, L)
+    g = rng.standard_normal(n * n)
+    gY, gat, gar, gcp = oracle.das_ext_adjoint(Y, Gt, Gr, Ro, 0.02, 0.1, 1500.0, g, method,
+                                               "linear", at, ar, nthreads=8)
+    plan = DasPlan(method, K, Ro, Gt, Gr, precision="f64")
+    Yt = torch.tensor(Y.T.copy(), device="cuda")
+    att, art = torch.tensor(at, device="cuda"), torch.tensor(ar, device="cuda")
+    gt = torch.tensor(g, device="cuda")
+    try:
+        for tune in (0, 1, 1):
+            _lib.lib().dream_hip_set_autotune(tune)
+            rY, rat, rar, rcp = plan.adjoint(Yt, gt, 0.02, 0.1, 1500.0, att, art, "linear")
+            assert rel_l2(rY.cpu().numpy().T, gY) <= 1e-10
+            assert rel_l2(rat.cpu().numpy(), gat) <= 1e-10
+            assert rel_l2(rar.cpu().numpy(), gar) <= 1e-10
+            assert abs(float(rcp) - gcp) <= 1e-9 * abs(gcp)
+    finally:
+        _lib.lib().dream_hip_set_autotune(1)
+
+
 def test_autotuned_shapes_give_the_same_image(gpu):
     """The forward launch shape picked by the autotuner (waves per work-group,
     split) changes only the summation order: ramp data give identical images
