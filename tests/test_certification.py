@@ -1,0 +1,126 @@
+"""The error bounds behind the kernels' certified fast index paths, checked
+by emulating their arithmetic on the CPU (correctly rounded FMAs through
+exact rationals) with an rsqrt seed at the worst relative error measured on
+gfx950 (2^-24.2) and beyond it (2^-23).
+
+SIR (sir.hip, `sir_estimate`): s = fma(rx, rx, ry^2 + rz^2), one
+Goldschmidt step, u = fma(r, c1, -c2); rint(u) is kept when
+|u - rint(u)| <= 1/2 - 2^-36 (nt + 2 + |c2|).  The exact index is the
+reference's rint(((sqrt((rx*rx + ry*ry) + rz*rz) * 1000) / cp - d) / dt)
+(src/dreamrect.cc:306-314).
+
+SAFT (das.hip, `saft_pos`): u = (r / cp + delay) (2 fs) with the fast r; it
+is kept when it lies farther than 2^-29 |u| + 2^-40 (|2 fs delay| + 1) from
+an integer.  The exact position is the reference's 2 (sqrt(s) / cp + delay)
+fs truncated (extra_src/das.cc:329-335).
+
+A mistake in either bound (a dropped term, a margin too small for the
+seed's error) makes a certified index differ from the exact one here, on
+inputs placed next to rounding boundaries.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+SEED_ERRORS = (2.0 ** -24.2, -(2.0 ** -24.2), 2.0 ** -23, -(2.0 ** -23))
+
+
+def fma(a, b, c):
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def fast_root(s, e0):
+    """r and 1/(2r) from a seed with relative error e0 and one Goldschmidt step."""
+    y = (1.0 / math.sqrt(s)) * (1.0 + e0)
+    g = s * y
+    h = 0.5 * y
+    e = fma(-g, h, 0.5)
+    return fma(g, e, g), fma(h, e, h)
+
+
+def test_goldschmidt_step_error():
+    """One step from a 2^-24.2 seed leaves r within 2^-47.5 (relative)."""
+    rng = np.random.default_rng(1)
+    for s in rng.uniform(1e-2, 1e4, 2000):
+        for e0 in SEED_ERRORS[:2]:
+            g, h = fast_root(float(s), e0)
+            r = math.sqrt(s)
+            assert abs(g - r) <= 2.0 ** -47.5 * r
+            assert abs(2.0 * h * r - 1.0) <= 2.0 ** -47.0
+
+
+def _sir_cases(rng, n):
+    """(rx, ry, rz, d, dt, cp, nt) with the exact index's argument put within
+    1e-13 .. 1e-5 of a half-integer (a rint boundary) for half of them."""
+    out = []
+    for i in range(n):
+        cp, dt, nt = 1500.0, float(rng.choice([0.02, 0.025, 0.05])), int(rng.integers(100, 4000))
+        rx, ry, rz = rng.uniform(-8, 8), rng.uniform(-8, 8), rng.uniform(1, 80)
+        r = math.sqrt((rx * rx + ry * ry) + rz * rz)
+        t = (r * 1000.0) / cp
+        k = float(rng.integers(0, nt))
+        d = t - (k + 0.5) * dt if i % 2 else t - rng.uniform(0, nt) * dt
+        d += rng.choice([-1.0, 1.0]) * 10.0 ** rng.uniform(-13, -5) * dt
+        out.append((rx, ry, rz, d, dt, cp, nt))
+    return out
+
+
+def test_sir_certified_index_equals_exact():
+    rng = np.random.default_rng(2)
+    n_cert = n_near = 0
+    for rx, ry, rz, d, dt, cp, nt in _sir_cases(rng, 1500):
+        ryy, rzz = ry * ry, rz * rz
+        exact = float(np.rint((((math.sqrt((rx * rx + ryy) + rzz) * 1000.0) / cp) - d) / dt))
+        c1, c2 = 1000.0 / (cp * dt), d * (1.0 / dt)
+        lim = 0.5 - 2.0 ** -36 * (nt + 2.0 + abs(c2))
+        for e0 in SEED_ERRORS:
+            g, _ = fast_root(fma(rx, rx, ryy + rzz), e0)
+            u = fma(g, c1, -c2)
+            ru = float(np.rint(u))
+            if abs(u - ru) <= lim:
+                n_cert += 1
+                if abs(u) <= nt + 2:
+                    assert ru == exact, (rx, ry, rz, d, dt, e0)
+                else:                               # out of range either way
+                    assert not (0 <= exact < nt)
+            else:
+                n_near += 1
+    assert n_cert > 3500 and n_near > 500          # both paths, near boundaries too
+
+
+@pytest.mark.parametrize("delay", [0.0, 0.37, -0.8])
+def test_saft_certified_position_equals_exact(delay):
+    rng = np.random.default_rng(3)
+    dt, cp = 0.02, 1500.0
+    fs = (1.0 / dt) * 1000.0
+    inv_cp, dl = 1.0 / cp, delay / 1000.0
+    fs2 = fs + fs
+    bnd = 2.0 ** -40 * (abs(2.0 * fs * dl) + 1.0)
+    n_cert = n_near = 0
+    for i in range(1500):
+        dx, dy = rng.uniform(-20, 20), rng.uniform(-5, 5)
+        dz = rng.uniform(1, 60)
+        if i % 2:                                   # put u next to an integer
+            r = math.sqrt((dx * dx + dy * dy) + dz * dz)
+            n = math.floor((r * inv_cp + dl) * fs2)
+            target = (n / fs2 - dl) / inv_cp
+            dz = math.sqrt(max(target * target - dx * dx - dy * dy, 1.0))
+            dz *= 1.0 + rng.choice([-1.0, 1.0]) * 10.0 ** rng.uniform(-15, -7)
+        r = math.sqrt((dx * dx + dy * dy) + dz * dz)
+        t = r * inv_cp + dl
+        u_exact = (t + t) * fs
+        for e0 in SEED_ERRORS:
+            g, _ = fast_root(fma(dx, dx, fma(dy, dy, dz * dz)), e0)
+            u = (g * inv_cp + dl) * fs2
+            # The stated bound holds for the measured seed error; a 2^-23 seed
+            # (beyond it) still stays within 2^-44, far inside the margin.
+            bound = 2.0 ** -46 if abs(e0) <= 2.0 ** -24.2 else 2.0 ** -44
+            assert abs(u - u_exact) <= bound * (abs(u_exact) + abs(2.0 * fs * dl) + 1.0)
+            if abs(u - round(u)) > fma(abs(u), 2.0 ** -29, bnd):
+                n_cert += 1
+                assert math.floor(u) == math.floor(u_exact) and int(u) == int(u_exact)
+            else:
+                n_near += 1
+    assert n_cert > 3500 and n_near > 500
