@@ -438,6 +438,16 @@ int dream_hip_rect_indices(double xo, double yo, double zo, double a, double b, 
   });
 }
 
+int dream_hip_rsq_seed(const double *s, int64_t n, double *y)
+{
+  return guarded([&]() {
+    require(n >= 0, "rsq_seed: negative count");
+    require(n == 0 || (s && y), "rsq_seed: null array");
+    rsq_seed(s, n, y);
+    return DREAM_HIP_OK;
+  });
+}
+
 int dream_hip_sir_plan_create(const DreamSirProblem *problems, int32_t n, int out_dtype,
                               void **plan)
 {

@@ -312,6 +312,14 @@ __global__ void rect_indices_kernel(double xo, double yo, double zo, const doubl
                                   dt, h);
 }
 
+// The hardware FP64 rsqrt seed of each s (the first step of sir_estimate
+// and saft_pos), for tests of the accuracy the certification assumes.
+__global__ void rsq_seed_kernel(const double *s, int64_t n, double *y)
+{
+  const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = __builtin_amdgcn_rsq(s[i]);
+}
+
 int gcd_int(int a, int b)
 {
   while (b) { int t = a % b; a = b; b = t; }
@@ -829,6 +837,20 @@ void SirPlan::status(int32_t *codes, int64_t *oob)
     if (codes) codes[i] = code;
     if (oob) oob[i] = (int64_t) st[i].oob;
   }
+}
+
+void rsq_seed(const double *s, int64_t n, double *y)
+{
+  if (n <= 0) return;
+  double *d = nullptr;
+  DREAM_HIP_CHECK(hipMalloc(&d, 2 * (size_t) n * sizeof(double)));
+  DREAM_HIP_CHECK(hipMemcpy(d, s, (size_t) n * sizeof(double), hipMemcpyHostToDevice));
+  const int blocks = (int) ((n + 255) / 256);
+  hipLaunchKernelGGL(rsq_seed_kernel, dim3(blocks), dim3(256), 0, lib_stream(), d, n, d + n);
+  DREAM_HIP_CHECK(hipGetLastError());
+  DREAM_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  DREAM_HIP_CHECK(hipMemcpy(y, d + n, (size_t) n * sizeof(double), hipMemcpyDeviceToHost));
+  (void) hipFree(d);
 }
 
 void rect_indices(double xo, double yo, double zo, double a, double b, double dx, double dy,

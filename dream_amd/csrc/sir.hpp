@@ -148,6 +148,7 @@ void run_arr_annu(const AnnuProblem &a, int out_dtype, hipStream_t s, int *sir_e
 
 // Device parity helper: per-contribution sample indices of one
 // (point, element) pair from the same device index routine.
+void rsq_seed(const double *s, int64_t n, double *y);
 void rect_indices(double xo, double yo, double zo, double a, double b, double dx, double dy,
                   double dt, double delay, double cp, int64_t *it_out, int64_t cap,
                   int64_t *count);

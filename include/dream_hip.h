@@ -239,6 +239,12 @@ int dream_hip_rect_indices(double xo, double yo, double zo, double a, double b, 
                            double dy, double dt, double delay, double cp, int64_t *it_out,
                            int64_t cap, int64_t *count);
 
+/* The device's FP64 reciprocal-square-root seed of each s[i] (host arrays of
+ * n doubles).  The SIR and SAFT index paths certify their fast estimates
+ * with margins that assume a seed relative error near 2^-24; a test hook
+ * for that assumption, with no counterpart in the reference. */
+int dream_hip_rsq_seed(const double *s, int64_t n, double *y);
+
 /* Batched uneven SIR: many dream_arr_rect problems in one persistent launch.
  * Ro, delay and h are DEVICE pointers; G, focal, apod are HOST pointers. */
 typedef struct DreamSirProblem {

@@ -229,3 +229,19 @@ def test_autotuned_batch_equals_untuned(gpu):
                 assert torch.allclose(a, b, rtol=1e-12, atol=1e-12 * float(b.abs().max()))
     finally:
         _lib.lib().dream_hip_set_autotune(1)
+
+
+def test_rsqrt_seed_accuracy(gpu):
+    """The hardware FP64 rsqrt seed that the SIR and SAFT fast index paths
+    start from: its relative error must stay within 2^-23 (the CPU emulation
+    in test_certification.py shows both certifications hold up to there; the
+    margins were sized on the 2^-24.2 measured on MI355X).  Mantissas dense
+    over [1, 4) (one binade of s per binade of the result), exponents spread."""
+    from dream_amd import _lib
+    m = np.linspace(1.0, 4.0, 1 << 20, endpoint=False)
+    s = np.concatenate([m * 2.0 ** e for e in (-20, -4, 0, 6, 12, 30)])
+    y = np.empty_like(s)
+    _lib.check(_lib.lib().dream_hip_rsq_seed(s.ctypes.data, s.size, y.ctypes.data))
+    rel = np.abs(y * np.sqrt(s) - 1.0)
+    assert rel.max() <= 2.0 ** -23, np.log2(rel.max())
+
