@@ -214,6 +214,40 @@ def test_frames_equal_single_calls(gpu):
         assert torch.equal(Im[f], single)
 
 
+@pytest.mark.parametrize("method,want_gy", [("saft", True), ("rca_rowtx", True),
+                                            ("rca_rowtx", False)])
+def test_adjoint_frames_equal_single_calls(gpu, method, want_gy):
+    """Frames in one adjoint launch: each frame's gY is its own single-frame
+    gY; the apodization and cp gradients are the sums over the frames."""
+    from dream_amd.torch_ops import DasPlan
+    Y, Gt, Gr, Ro, rng = _setup(method, seed=9, No=1500)
+    F = 3
+    Yf = torch.tensor(np.stack([Y.T * (f + 1) for f in range(F)]), device="cuda").contiguous()
+    g = torch.tensor(rng.standard_normal((F, Ro.shape[0])), device="cuda").contiguous()
+    plan = DasPlan(method, Y.shape[0], Ro, Gt, Gr, precision="f64")
+    at = torch.tensor(rng.uniform(0.5, 1.5, Gt.shape[0]), device="cuda")
+    ar = None if method == "saft" else torch.tensor(rng.uniform(0.5, 1.5, Gr.shape[0]),
+                                                     device="cuda")
+    interp = "linear" if want_gy else "nearest"
+    gY, gat, gar, gcp = plan.adjoint(Yf, g, 0.02, 0.1, 1500.0, at, ar, interp, want_gy=want_gy,
+                                     want_cp=want_gy)
+    sat, sar, scp = 0.0, 0.0, 0.0
+    for f in range(F):
+        y1, a1, r1, c1 = plan.adjoint(Yf[f].contiguous(), g[f].contiguous(), 0.02, 0.1, 1500.0,
+                                      at, ar, interp, want_gy=want_gy, want_cp=want_gy)
+        if want_gy:
+            assert rel_l2(gY[f].cpu().numpy(), y1.cpu().numpy()) <= 1e-12
+            scp = scp + float(c1)
+        sat = sat + a1.cpu().numpy()
+        if ar is not None:
+            sar = sar + r1.cpu().numpy()
+    assert rel_l2(gat.cpu().numpy(), sat) <= 1e-12
+    if ar is not None:
+        assert rel_l2(gar.cpu().numpy(), sar) <= 1e-12
+    if want_gy:
+        assert abs(float(gcp) - scp) <= 1e-10 * abs(scp)
+
+
 def test_autograd_function(gpu, oracle):
     from dream_amd.torch_ops import DasPlan, das_diff
     method = "saft"
