@@ -146,6 +146,34 @@ def test_long_sir_global_mode(gpu, oracle):
     assert np.max(np.abs(H - ref)) <= 1e-12 * np.abs(ref).max()
 
 
+def test_batch_mixing_global_and_lds_problems(gpu, oracle):
+    """One plan whose problems use both accumulation modes (nt beyond and
+    within the LDS histogram), FP64 and FP32, against the oracle."""
+    from dream_amd.torch_ops import SirBatch
+    rng = np.random.default_rng(51)
+    specs = [(12000, 0.005), (400, 0.025), (9000, 0.005), (700, 0.02)]
+    probs, refs = [], []
+    for nt, dt in specs:
+        L, No = 3, 40
+        G = np.c_[np.linspace(-1.5, 1.5, L), np.zeros(L), np.zeros(L)]
+        Ro = np.c_[rng.uniform(-2, 2, No), rng.uniform(-1, 1, No), rng.uniform(8, 12, No)]
+        d = Ro[:, 2] * 1000.0 / 1500.0 - 1.0
+        probs.append(dict(Ro=torch.tensor(Ro.T.copy(), device="cuda"), a=0.6, b=2.0, dx=0.05,
+                          dy=0.05, dt=dt, nt=nt, delay=torch.tensor(d, device="cuda"), v=1.0,
+                          cp=1500.0, G=G))
+        ref, _, _ = oracle.dream_arr_rect(Ro, 0.6, 2.0, 0.05, 0.05, dt, nt, d, 1.0, 1500.0, G,
+                                          err_level="ignore", nthreads=4)
+        refs.append(ref)
+    for dtype, tol in ((torch.float64, 1e-12), (torch.float32, 2e-5)):
+        for H, ref in zip(SirBatch(probs, out_dtype=dtype).run(), refs):
+            torch.cuda.synchronize()
+            Hn = H.double().cpu().numpy().T
+            if dtype == torch.float64:
+                assert np.max(np.abs(Hn - ref)) <= tol * np.abs(ref).max()
+            else:
+                assert rel_l2(Hn, ref) <= tol
+
+
 def test_error_codes_gpu(gpu):
     from dream_amd import dream_arr_rect, dreamrect
     Ro = np.array([[0.0, 0.0, 20.0], [0.0, 0.0, 21.0], [0.0, 0.0, 22.0]])
