@@ -29,6 +29,7 @@ EXPORTS = (
     "dream_hip_last_error", "dream_hip_version", "dream_hip_device_count",
     "dream_hip_set_autotune",
     "dream_hip_abort", "dream_hip_set_running", "dream_hip_is_running",
+    "dream_hip_test_chunking",
     "dream_hip_arr_rect", "dream_hip_rect", "dream_hip_arr_rect_dev",
     "dream_hip_rect_indices", "dream_hip_rsq_seed",
     "dream_hip_sir_plan_create", "dream_hip_sir_plan_run",
@@ -101,6 +102,8 @@ def _declare(L):
     L.dream_hip_set_running.argtypes = []
     L.dream_hip_is_running.restype = _i32
     L.dream_hip_is_running.argtypes = []
+    L.dream_hip_test_chunking.restype = None
+    L.dream_hip_test_chunking.argtypes = [_i64, _i64]
     sir_common = [_vp, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, _i64, _i32, _vp, _dbl, _dbl]
     L.dream_hip_arr_rect.restype = _i32
     L.dream_hip_arr_rect.argtypes = [_dbl] + sir_common + [
@@ -212,11 +215,19 @@ def lib():
     return _lib
 
 
+ECANCELLED = -5
+
+
+class Cancelled(RuntimeError):
+    """dream_hip_abort() stopped the call (DREAM_HIP_ECANCELLED)."""
+
+
 def check(rc):
-    """Raise RuntimeError carrying the library's message for a non-zero code."""
+    """Raise RuntimeError carrying the library's message for a non-zero code
+    (Cancelled for a call stopped by dream_hip_abort)."""
     if rc != OK:
         msg = lib().dream_hip_last_error().decode(errors="replace")
-        raise RuntimeError(f"dream_hip error {rc}: {msg}")
+        raise (Cancelled if rc == ECANCELLED else RuntimeError)(f"dream_hip error {rc}: {msg}")
     return rc
 
 

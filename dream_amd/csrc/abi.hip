@@ -32,6 +32,52 @@ void hip_fail(hipError_t e, const char *what, const char *file, int line)
 
 bool cancelled() { return !g_running.load(std::memory_order_relaxed); }
 
+namespace {
+std::atomic<int64_t> g_chunk_work(0);          // 0: the default
+std::atomic<int64_t> g_cancel_after(0);        // > 0: abort after that many chunks
+} // namespace
+
+int64_t chunk_work()
+{
+  const int64_t w = g_chunk_work.load(std::memory_order_relaxed);
+  return w > 0 ? w : (int64_t) 10000000000ll;
+}
+
+// A gate costs nothing until a call is actually cut into chunks: the capture
+// test and the two events are set up at the first chunk.
+ChunkGate::ChunkGate(hipStream_t s) : s_(s) {}
+
+ChunkGate::~ChunkGate()
+{
+  for (auto &e : ev_)
+    if (e) (void) hipEventDestroy(e);
+}
+
+void ChunkGate::before()
+{
+  if (n_ == 0 && !ev_[0] && !capturing_) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    DREAM_HIP_CHECK(hipStreamIsCapturing(s_, &st));
+    capturing_ = st != hipStreamCaptureStatusNone;
+    if (!capturing_)
+      for (auto &e : ev_) DREAM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  if (capturing_) return;
+  if (n_ >= 2) DREAM_HIP_CHECK(hipEventSynchronize(ev_[n_ & 1]));
+  if (cancelled()) throw Error(DREAM_HIP_ECANCELLED, "cancelled between work chunks");
+}
+
+void ChunkGate::after()
+{
+  if (capturing_) return;
+  DREAM_HIP_CHECK(hipEventRecord(ev_[n_ & 1], s_));
+  n_++;
+  int64_t c = g_cancel_after.load();
+  while (c > 0 && !g_cancel_after.compare_exchange_weak(c, c - 1)) {
+  }
+  if (c == 1) g_running.store(false);
+}
+
 hipStream_t lib_stream()
 {
   if (!g_stream) DREAM_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
@@ -221,6 +267,12 @@ int dream_hip_device_count(void)
 }
 
 void dream_hip_set_autotune(int on) { set_autotune(on != 0); }
+
+void dream_hip_test_chunking(int64_t work_per_chunk, int64_t cancel_after_chunks)
+{
+  g_chunk_work.store(work_per_chunk > 0 ? work_per_chunk : 0);
+  g_cancel_after.store(cancel_after_chunks > 0 ? cancel_after_chunks : 0);
+}
 
 void dream_hip_abort(int) { g_running.store(false); }
 void dream_hip_set_running(void) { g_running.store(true); }

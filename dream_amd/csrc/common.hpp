@@ -53,6 +53,32 @@ struct Status {
 
 bool cancelled();
 
+// Work units (pixel-channel pairs, sub-element contributions) per chunk of a
+// long call: about 4 ms of device time (default 10^10), or the value a
+// test set through dream_hip_test_chunking.
+int64_t chunk_work();
+
+// Runs a long call as consecutive chunk launches on stream `s`, testing the
+// cancellation flag before each one.  At most two chunks are queued ahead of
+// the device (before chunk i the host waits for chunk i - 2), so a call
+// cancelled from another thread stops within about two chunks and throws
+// DREAM_HIP_ECANCELLED; the chunks already launched have written their
+// outputs.  While `s` captures a graph nothing is waited for or tested.
+class ChunkGate {
+public:
+  explicit ChunkGate(hipStream_t s);
+  ~ChunkGate();
+  ChunkGate(const ChunkGate &) = delete;
+  ChunkGate &operator=(const ChunkGate &) = delete;
+  void before();   // call before launching a chunk
+  void after();    // call after launching it
+private:
+  hipStream_t s_;
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+  int64_t n_ = 0;
+  bool capturing_ = false;
+};
+
 // Thread-local stream of the library (created on first use, default
 // priority).
 hipStream_t lib_stream();

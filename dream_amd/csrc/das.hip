@@ -84,6 +84,9 @@ __device__ __forceinline__ double wave_sum_d(double v)
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+// Wave sum in the data type: FP64 plans reduce in FP64 all the way.
+__device__ __forceinline__ float wave_sum_t(float v) { return wave_sum_f(v); }
+__device__ __forceinline__ double wave_sum_t(double v) { return wave_sum_d(v); }
 
 __device__ __forceinline__ int64_t pixel_of(const int32_t *perm, int64_t idx)
 {
@@ -285,8 +288,8 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
     GT gy_c = GT(0), gx_r = GT(0);
     if (TYPE == DAS_RCA_COLTX || TYPE == DAS_RCA_ROWTX) {
       const bool given = A.stripe_given != 0;
-      gy_c = stripe_gap<GT>(yo, given ? (GT) A.sy_lo : Gr[Lr], given ? (GT) A.sy_hi : Gr[2 * Lr - 1]);
-      gx_r = stripe_gap<GT>(xo, given ? (GT) A.sx_lo : Gt[0], given ? (GT) A.sx_hi : Gt[Lt - 1]);
+      gy_c = stripe_gap<GT>(yo, given ? (GT) A.sy_lo : Gr[Lt], given ? (GT) A.sy_hi : Gr[2 * Lt - 1]);
+      gx_r = stripe_gap<GT>(xo, given ? (GT) A.sx_lo : Gt[0], given ? (GT) A.sx_hi : Gt[Lr - 1]);
     }
     const bool delay_rx = A.delay_rx != 0;
     const int nchunks = (Lr + kRC - 1) / kRC;
@@ -377,9 +380,9 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
   // and A-scan indices live in SGPRs.
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t idx = (int64_t) blockIdx.x * 64 + lane;
+  const int64_t idx = ((int64_t) blockIdx.x + A.block0) * 64 + lane;
   const int sidx = blockIdx.y;
-  const int64_t frame = blockIdx.z;
+  const int64_t frame = (int64_t) blockIdx.z + A.frame0;
   const bool valid = idx < A.No;
   const int64_t p = valid ? pixel_of(perm, idx) : 0;
   bool oob = false;
@@ -552,9 +555,9 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj, int
   __shared__ double s_zc[kThreads / 64];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t region = blockIdx.x;
+  const int64_t region = (int64_t) blockIdx.x + A.block0;
   const int grp = blockIdx.y;
-  const int64_t frame = blockIdx.z;
+  const int64_t frame = (int64_t) blockIdx.z + A.frame0;
   const int Lt = A.Lt, Lr = A.Lr;
   const int K = (int) A.K;
   const bool want_gy = A.gY != nullptr;
@@ -575,7 +578,7 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj, int
   const DT *ar = (const DT *) A.apod_r;
   GT y_lo = 0, y_hi = 0, x_lo = 0, x_hi = 0;
   if (TYPE == DAS_RCA_COLTX || TYPE == DAS_RCA_ROWTX) {
-    y_lo = Gr[Lr]; y_hi = Gr[2 * Lr - 1]; x_lo = Gt[0]; x_hi = Gt[Lt - 1];
+    y_lo = Gr[Lt]; y_hi = Gr[2 * Lt - 1]; x_lo = Gt[0]; x_hi = Gt[Lr - 1];
   }
 
   // The work-group covers `subs` consecutive regions, one after the other,
@@ -846,12 +849,12 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
   const DT *ar = (const DT *) A.apod_r;
   const int K = (int) A.K;
   const int64_t tiles_per_frame = (A.No + kThreads - 1) / kThreads;
-  const GT y_lo = Gr[Lr], y_hi = Gr[2 * Lr - 1], x_lo = Gt[0], x_hi = Gt[Lt - 1];
+  const GT y_lo = Gr[Lt], y_hi = Gr[2 * Lt - 1], x_lo = Gt[0], x_hi = Gt[Lr - 1];
   const GT tmax = (GT) A.leg_max;
   double zcp = 0.0;
   const bool lane0 = (threadIdx.x & 63) == 0;
 
-  for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+  for (int64_t tile = A.block0 + blockIdx.x; tile < n_tiles; tile += gridDim.x) {
     const int64_t frame = tile / tiles_per_frame;
     const int64_t idx = (tile - frame * tiles_per_frame) * kThreads + threadIdx.x;
     const bool valid = idx < A.No;
@@ -906,15 +909,15 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
           zj[jj] += wi * val;
           if (INTERP == DREAM_HIP_LINEAR) zc += wi * wj * slope * (DT) (tt0 + tr[jj]);
         }
-        const float zsum = wave_sum_f((float) (gp * zi));
-        if (lane0 && zsum != 0.0f) atomicAdd(&zt[i], (double) zsum);
+        const DT zsum = wave_sum_t(gp * zi);
+        if (lane0 && zsum != DT(0)) atomicAdd(&zt[i], (double) zsum);
         if (INTERP == DREAM_HIP_LINEAR) zcp += (double) (gp * zc);
       }
 #pragma unroll
       for (int jj = 0; jj < kRC; jj++) {
         if (j0 + jj < Lr) {
-          const float zsum = wave_sum_f((float) (gp * zj[jj]));
-          if (lane0 && zsum != 0.0f) atomicAdd(&zr[j0 + jj], (double) zsum);
+          const DT zsum = wave_sum_t(gp * zj[jj]);
+          if (lane0 && zsum != DT(0)) atomicAdd(&zr[j0 + jj], (double) zsum);
         }
       }
     }
@@ -986,6 +989,15 @@ void zero_async(void *p, size_t bytes, hipStream_t s)
   DREAM_HIP_CHECK(hipGetLastError());
 }
 
+// The reference reads the RCA stripe extents at rows[Lt], rows[2 Lt - 1]
+// (y column of the rows, offset by the column count) and columns[Lr - 1]
+// (extra_src/das.cc:464-467, :560-563); those entries exist in the 3 x L
+// tables only when 2 Lt <= 3 Lr and Lr <= 3 Lt.
+bool rca_extents_in_bounds(int64_t Lt, int64_t Lr)
+{
+  return 2 * Lt <= 3 * Lr && Lr <= 3 * Lt;
+}
+
 int cu_count()
 {
   static int cus = 0;
@@ -1047,21 +1059,69 @@ std::vector<FwdShape> forward_candidates(int type, int64_t No, int64_t n_frames,
 }
 
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
-void fwd_launch(const DasArgs &a, hipStream_t s);
+void fwd_launch(const DasArgs &a, hipStream_t s, ChunkGate *gate);
+
+// Pixel-channel pairs per pixel of one frame.
+inline int64_t pairs_per_pixel(int type, int64_t Lt, int64_t Lr)
+{
+  return type == DAS_SAFT ? Lt : Lt * Lr;
+}
+
+// Runs the (blocks x frames) grid of a launch as chunks of about
+// chunk_work() pairs, `work_per_block` pairs each block: whole frames per
+// chunk when a frame is small, else block ranges within a frame.  `launch`
+// takes (block0, n_blocks, frame0, n_frames); a gate tests cancellation
+// between chunks (none when the call is one chunk).
+template <class F>
+void run_chunked(int64_t blocks, int64_t frames, int64_t work_per_block, ChunkGate *gate,
+                 F launch)
+{
+  const int64_t per_chunk = std::max<int64_t>(1, chunk_work() / std::max<int64_t>(work_per_block, 1));
+  if (!gate || per_chunk >= blocks * frames) {
+    launch((int64_t) 0, blocks, (int64_t) 0, frames);
+    return;
+  }
+  if (per_chunk >= blocks) {
+    const int64_t fpc = per_chunk / blocks;
+    for (int64_t f0 = 0; f0 < frames; f0 += fpc) {
+      gate->before();
+      launch((int64_t) 0, blocks, f0, std::min(fpc, frames - f0));
+      gate->after();
+    }
+    return;
+  }
+  for (int64_t f0 = 0; f0 < frames; f0++)
+    for (int64_t b0 = 0; b0 < blocks; b0 += per_chunk) {
+      gate->before();
+      launch(b0, std::min(per_chunk, blocks - b0), f0, (int64_t) 1);
+      gate->after();
+    }
+}
 
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
-void fwd_launch_shape(const DasArgs &a, FwdShape sh, hipStream_t s)
+void fwd_launch_shape(const DasArgs &a0, FwdShape sh, hipStream_t s, ChunkGate *gate = nullptr)
 {
-  if (sh.split > 1 && (size_t) sh.split * a.n_frames * a.No * sizeof(DT) > a.scratch_bytes)
+  if (sh.split > 1 && (size_t) sh.split * a0.n_frames * a0.No * sizeof(DT) > a0.scratch_bytes)
     sh.split = 1;
-  dim3 grid((unsigned) ((a.No + 63) / 64), (unsigned) sh.split, (unsigned) a.n_frames);
   dim3 block((unsigned) (64 * sh.waves));
-  if (a.status)
-    hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, true>), grid, block, 0, s, a,
-                       a.perm, sh.split, (DT *) a.scratch);
-  else
-    hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, false>), grid, block, 0, s, a,
-                       a.perm, sh.split, (DT *) a.scratch);
+  const int64_t tiles = (a0.No + 63) / 64;
+  auto launch = [&](int64_t b0, int64_t nb, int64_t f0, int64_t nf) {
+    DasArgs a = a0;
+    a.block0 = b0;
+    a.frame0 = f0;
+    dim3 grid((unsigned) nb, (unsigned) sh.split, (unsigned) nf);
+    if (a.status)
+      hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, true>), grid, block, 0, s,
+                         a, a.perm, sh.split, (DT *) a.scratch);
+    else
+      hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, false>), grid, block, 0, s,
+                         a, a.perm, sh.split, (DT *) a.scratch);
+    DREAM_HIP_CHECK(hipGetLastError());
+  };
+  // Split launches are small images (the grid cannot fill the device): one chunk.
+  run_chunked(tiles, a0.n_frames, 64 * pairs_per_pixel(TYPE, a0.Lt, a0.Lr),
+              sh.split > 1 ? nullptr : gate, launch);
+  const DasArgs &a = a0;
   if (sh.split > 1) {
     const int64_t n = a.No * a.n_frames;
     hipLaunchKernelGGL(das_reduce_kernel<DT>, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0,
@@ -1073,7 +1133,7 @@ void fwd_launch_shape(const DasArgs &a, FwdShape sh, hipStream_t s)
 // The heuristic shape, or the autotuner's pick for this shape class (not
 // when the launch also collects the out-of-bounds status).
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
-void fwd_launch(const DasArgs &a, hipStream_t s)
+void fwd_launch(const DasArgs &a, hipStream_t s, ChunkGate *gate)
 {
   const std::vector<FwdShape> cand = forward_candidates(TYPE, a.No, a.n_frames, a.Lt, a.Lr);
   int pick = 0;
@@ -1087,34 +1147,36 @@ void fwd_launch(const DasArgs &a, hipStream_t s)
                            s);
     if (t >= 0) pick = t;
   }
-  fwd_launch_shape<DT, GT, TYPE, INTERP, APOD>(a, cand[(size_t) pick], s);
+  fwd_launch_shape<DT, GT, TYPE, INTERP, APOD>(a, cand[(size_t) pick], s, gate);
 }
 
 template <typename DT, typename GT, int TYPE>
-void fwd_dispatch_interp(const DasArgs &a, int interp, bool apod, hipStream_t s)
+void fwd_dispatch_interp(const DasArgs &a, int interp, bool apod, hipStream_t s, ChunkGate *g)
 {
   if (interp == DREAM_HIP_NEAREST) {
-    if (apod) fwd_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, true>(a, s);
-    else fwd_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, false>(a, s);
+    if (apod) fwd_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, true>(a, s, g);
+    else fwd_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, false>(a, s, g);
   } else {
-    if (apod) fwd_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, true>(a, s);
-    else fwd_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, false>(a, s);
+    if (apod) fwd_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, true>(a, s, g);
+    else fwd_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, false>(a, s, g);
   }
 }
 
 template <typename DT, typename GT>
-void fwd_dispatch_type(const DasArgs &a, int type, int interp, bool apod, hipStream_t s)
+void fwd_dispatch_type(const DasArgs &a, int type, int interp, bool apod, hipStream_t s,
+                       ChunkGate *g)
 {
   switch (type) {
-  case DAS_SAFT: fwd_dispatch_interp<DT, GT, DAS_SAFT>(a, interp, apod, s); break;
-  case DAS_TFM: fwd_dispatch_interp<DT, GT, DAS_TFM>(a, interp, apod, s); break;
-  case DAS_RCA_COLTX: fwd_dispatch_interp<DT, GT, DAS_RCA_COLTX>(a, interp, apod, s); break;
-  default: fwd_dispatch_interp<DT, GT, DAS_RCA_ROWTX>(a, interp, apod, s); break;
+  case DAS_SAFT: fwd_dispatch_interp<DT, GT, DAS_SAFT>(a, interp, apod, s, g); break;
+  case DAS_TFM: fwd_dispatch_interp<DT, GT, DAS_TFM>(a, interp, apod, s, g); break;
+  case DAS_RCA_COLTX: fwd_dispatch_interp<DT, GT, DAS_RCA_COLTX>(a, interp, apod, s, g); break;
+  default: fwd_dispatch_interp<DT, GT, DAS_RCA_ROWTX>(a, interp, apod, s, g); break;
   }
 }
 
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
-void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t s)
+void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t s,
+                ChunkGate *gate)
 {
   if (TYPE == DAS_SAFT || want_gy) {
     // Region-major: (pixel region, A-scan group, frame) work-groups.
@@ -1139,16 +1201,24 @@ void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t
     const int na = TYPE == DAS_SAFT ? kSaftGroup : kGI * kGJ;
     const size_t lds = want_gy ? (size_t) na * kWin * sizeof(double) : 0;
     const int64_t n_z = TYPE == DAS_SAFT ? a.Lt : (int64_t) a.Lt * a.Lr;
-    auto run = [&](int subs, bool clear_z, hipStream_t st) {
+    auto run = [&](int subs, bool clear_z, hipStream_t st, ChunkGate *g) {
       if (clear_z) {
         zero_async(a.z_t, (size_t) n_z * sizeof(double), st);
         zero_async(a.z_cp, sizeof(double), st);
       }
       if (want_gy) zero_async(a.gY, (size_t) (a.n_frames * a.y_stride) * sizeof(DT), st);
       const int64_t regions = (regions1 + subs - 1) / subs;
-      dim3 grid((unsigned) regions, (unsigned) groups, (unsigned) a.n_frames);
-      hipLaunchKernelGGL((das_adj_region_kernel<DT, GT, TYPE, INTERP, APOD>), grid,
-                         dim3(kThreads), lds, st, a, perm, n_gj, subs);
+      const int64_t per_region = (int64_t) subs * kRegion * pairs_per_pixel(TYPE, a.Lt, a.Lr);
+      run_chunked(regions, a.n_frames, per_region, g,
+                  [&](int64_t b0, int64_t nb, int64_t f0, int64_t nf) {
+                    DasArgs c = a;
+                    c.block0 = b0;
+                    c.frame0 = f0;
+                    dim3 grid((unsigned) nb, (unsigned) groups, (unsigned) nf);
+                    hipLaunchKernelGGL((das_adj_region_kernel<DT, GT, TYPE, INTERP, APOD>), grid,
+                                       dim3(kThreads), lds, st, c, perm, n_gj, subs);
+                    DREAM_HIP_CHECK(hipGetLastError());
+                  });
     };
     int pick = 0;
     if (subs_cand.size() > 1) {
@@ -1158,7 +1228,7 @@ void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t
                     (long long) a.No, a.Lt, a.Lr, (long long) a.K, (long long) a.n_frames);
       const int before = autotune_cache_size();
       const int t = autotune(key, (int) subs_cand.size(),
-                             [&](int i, hipStream_t st) { run(subs_cand[(size_t) i], true, st); },
+                             [&](int i, hipStream_t st) { run(subs_cand[(size_t) i], true, st, nullptr); },
                              s);
       if (t >= 0) pick = t;
       // Candidates ran here: start the accumulators again.
@@ -1167,40 +1237,47 @@ void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t
         zero_async(a.z_cp, sizeof(double), s);
       }
     }
-    run(subs_cand[(size_t) pick], false, s);
+    run(subs_cand[(size_t) pick], false, s, gate);
   } else {
+    // Grid-stride over (frame, 256-pixel tile); a chunk is a range of tiles.
     const int64_t tiles = ((a.No + kThreads - 1) / kThreads) * a.n_frames;
-    const int grid = (int) std::min<int64_t>(tiles, (int64_t) cu_count() * 4);
     const size_t lds = (size_t) (a.Lt + a.Lr) * sizeof(double);
-    hipLaunchKernelGGL((das_adj_pixel_kernel<DT, GT, TYPE, INTERP, APOD>), dim3(grid),
-                       dim3(kThreads), lds, s, a, perm, tiles);
+    run_chunked(tiles, 1, (int64_t) kThreads * pairs_per_pixel(TYPE, a.Lt, a.Lr), gate,
+                [&](int64_t b0, int64_t nb, int64_t, int64_t) {
+                  DasArgs c = a;
+                  c.block0 = b0;
+                  const int grid = (int) std::min<int64_t>(nb, (int64_t) cu_count() * 4);
+                  hipLaunchKernelGGL((das_adj_pixel_kernel<DT, GT, TYPE, INTERP, APOD>),
+                                     dim3(grid), dim3(kThreads), lds, s, c, perm, b0 + nb);
+                  DREAM_HIP_CHECK(hipGetLastError());
+                });
   }
 }
 
 template <typename DT, typename GT, int TYPE>
 void adj_dispatch_interp(const DasArgs &a, const int32_t *perm, int interp, bool apod,
-                         bool want_gy, hipStream_t s)
+                         bool want_gy, hipStream_t s, ChunkGate *g)
 {
   if (interp == DREAM_HIP_NEAREST) {
-    if (apod) adj_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, true>(a, perm, want_gy, s);
-    else adj_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, false>(a, perm, want_gy, s);
+    if (apod) adj_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, true>(a, perm, want_gy, s, g);
+    else adj_launch<DT, GT, TYPE, DREAM_HIP_NEAREST, false>(a, perm, want_gy, s, g);
   } else {
-    if (apod) adj_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, true>(a, perm, want_gy, s);
-    else adj_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, false>(a, perm, want_gy, s);
+    if (apod) adj_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, true>(a, perm, want_gy, s, g);
+    else adj_launch<DT, GT, TYPE, DREAM_HIP_LINEAR, false>(a, perm, want_gy, s, g);
   }
 }
 
 template <typename DT, typename GT>
 void adj_dispatch_type(const DasArgs &a, const int32_t *perm, int type, int interp, bool apod,
-                       bool want_gy, hipStream_t s)
+                       bool want_gy, hipStream_t s, ChunkGate *g)
 {
   switch (type) {
-  case DAS_SAFT: adj_dispatch_interp<DT, GT, DAS_SAFT>(a, perm, interp, apod, want_gy, s); break;
-  case DAS_TFM: adj_dispatch_interp<DT, GT, DAS_TFM>(a, perm, interp, apod, want_gy, s); break;
+  case DAS_SAFT: adj_dispatch_interp<DT, GT, DAS_SAFT>(a, perm, interp, apod, want_gy, s, g); break;
+  case DAS_TFM: adj_dispatch_interp<DT, GT, DAS_TFM>(a, perm, interp, apod, want_gy, s, g); break;
   case DAS_RCA_COLTX:
-    adj_dispatch_interp<DT, GT, DAS_RCA_COLTX>(a, perm, interp, apod, want_gy, s);
+    adj_dispatch_interp<DT, GT, DAS_RCA_COLTX>(a, perm, interp, apod, want_gy, s, g);
     break;
-  default: adj_dispatch_interp<DT, GT, DAS_RCA_ROWTX>(a, perm, interp, apod, want_gy, s); break;
+  default: adj_dispatch_interp<DT, GT, DAS_RCA_ROWTX>(a, perm, interp, apod, want_gy, s, g); break;
   }
 }
 
@@ -1335,20 +1412,22 @@ template void pixel_order<double>(const double *, int64_t, int32_t *);
 template void pixel_order<float>(const float *, int64_t, int32_t *);
 
 void launch_das_forward(const DasArgs &a, int type, int prec, int interp, bool apod,
-                        hipStream_t s)
+                        hipStream_t s, ChunkGate *gate)
 {
-  if (prec == DREAM_HIP_F64) fwd_dispatch_type<double, double>(a, type, interp, apod, s);
-  else if (prec == DREAM_HIP_F32) fwd_dispatch_type<float, double>(a, type, interp, apod, s);
-  else fwd_dispatch_type<float, float>(a, type, interp, apod, s);
+  if (prec == DREAM_HIP_F64) fwd_dispatch_type<double, double>(a, type, interp, apod, s, gate);
+  else if (prec == DREAM_HIP_F32) fwd_dispatch_type<float, double>(a, type, interp, apod, s, gate);
+  else fwd_dispatch_type<float, float>(a, type, interp, apod, s, gate);
   DREAM_HIP_CHECK(hipGetLastError());
 }
 
 void launch_das_adjoint(const DasArgs &a, int type, int prec, int interp, bool apod,
-                        bool want_gy, hipStream_t s)
+                        bool want_gy, hipStream_t s, ChunkGate *gate)
 {
   const int32_t *perm = a.perm;
-  if (prec == DREAM_HIP_F64) adj_dispatch_type<double, double>(a, perm, type, interp, apod, want_gy, s);
-  else if (prec == DREAM_HIP_F32) adj_dispatch_type<float, double>(a, perm, type, interp, apod, want_gy, s);
+  if (prec == DREAM_HIP_F64)
+    adj_dispatch_type<double, double>(a, perm, type, interp, apod, want_gy, s, gate);
+  else if (prec == DREAM_HIP_F32)
+    adj_dispatch_type<float, double>(a, perm, type, interp, apod, want_gy, s, gate);
   else throw Error(DREAM_HIP_EINVAL, "the adjoint is not available for DREAM_HIP_F32_REF handles");
   DREAM_HIP_CHECK(hipGetLastError());
 }
@@ -1371,7 +1450,8 @@ void das_constants(DasArgs &a, int prec, double dt, double cp)
   a.leg_max = std::ldexp(1.0, 26) / a.fs_khz;
 }
 
-DasHandle::DasHandle(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, int precision)
+DasHandle::DasHandle(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, int precision,
+                     bool stripes_given)
   : type_(Lr == 0 ? DAS_SAFT : type), K_(K), No_(No), Lt_(Lt), Lr_(Lr), prec_(precision)
 {
   require(type >= DAS_SAFT && type <= DAS_RCA_ROWTX, "das: unknown DAS type");
@@ -1383,6 +1463,10 @@ DasHandle::DasHandle(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, in
   const int64_t n_ascans = type_ == DAS_SAFT ? Lt : Lt * Lr;
   require(n_ascans * K < (1ll << 40), "das: data too large");
   if (type_ != DAS_SAFT) require(Lr >= 1, "das: TFM/RCA need receive elements");
+  if ((type_ == DAS_RCA_COLTX || type_ == DAS_RCA_ROWTX) && !stripes_given)
+    require(rca_extents_in_bounds(Lt, Lr),
+          "das: RCA stripe extents are read at rows[Lt .. 2 Lt - 1] and columns[Lr - 1] "
+          "(extra_src/das.cc:464-467); need 2 Lt <= 3 Lr and Lr <= 3 Lt");
   DREAM_HIP_CHECK(hipMalloc(&d_status_, sizeof(Status)));
   DREAM_HIP_CHECK(hipHostMalloc(&h_status_, sizeof(Status)));
 }
@@ -1492,7 +1576,8 @@ int DasHandle::run_dev(const void *Y, const void *Ro, const void *Gt, const void
             im_stride, Im);
   if (want_status) zero_async(d_status_, sizeof(Status), s);
   else a.status = nullptr;
-  if (No_ > 0) launch_das_forward(a, type_, prec_, interp, apod, s);
+  ChunkGate gate(s);
+  if (No_ > 0) launch_das_forward(a, type_, prec_, interp, apod, s, &gate);
   if (!want_status) return E_NONE;
   DREAM_HIP_CHECK(hipMemcpyAsync(h_status_, d_status_, sizeof(Status), hipMemcpyDeviceToHost, s));
   DREAM_HIP_CHECK(hipStreamSynchronize(s));
@@ -1576,6 +1661,8 @@ void DasHandle::adjoint_dev(const void *Y, const void *Ro, const void *Gt, const
   const int64_t n_ascans = type_ == DAS_SAFT ? Lt_ : Lt_ * Lr_;
   if (y_stride <= 0) y_stride = n_ascans * K_;
   if (im_stride <= 0) im_stride = No_;
+  require((uint64_t) y_stride * elem_size(prec_) < 0xFFFFFFF0ull,
+          "das adjoint: one frame of Y must be smaller than 4 GiB");
   require(type_ == DAS_SAFT || gY || (size_t) (Lt_ + Lr_) * 8 <= 65536,
           "das adjoint: too many elements for the apodization accumulators");
   // Accumulators: Z per A-scan (region kernel) or z_t/z_r (pixel kernel), z_cp.
@@ -1601,7 +1688,8 @@ void DasHandle::adjoint_dev(const void *Y, const void *Ro, const void *Gt, const
   const bool channel = type_ == DAS_SAFT || gY != nullptr;
   a.z_t = channel ? (void *) Z : (void *) zt;
   a.z_r = zr;
-  if (No_ > 0) launch_das_adjoint(a, type_, prec_, interp, apod, gY != nullptr, s);
+  ChunkGate gate(s);
+  if (No_ > 0) launch_das_adjoint(a, type_, prec_, interp, apod, gY != nullptr, s, &gate);
   else if (gY) zero_async(gY, (size_t) (n_frames * y_stride) * elem_size(prec_), s);
   const int n = (int) std::max<int64_t>(std::max<int64_t>(Lt_, Lr_), 1);
   const int blocks = (n + 255) / 256;
@@ -1681,7 +1769,7 @@ DasUni::DasUni(int type, int64_t K, const double gt[3], const double gr[3], cons
                 Nz = count(Ro[2], Ro[5], Ro[8]);
   require(Nx >= 1 && Ny >= 1 && Nz >= 1 && Lt_ >= 1, "das_uni: empty grid");
   No_ = Nx * Ny * Nz;
-  h_ = new DasHandle(type, K, No_, Lt_, Lr_, precision);
+  h_ = new DasHandle(type, K, No_, Lt_, Lr_, precision, true);
   // Device geometry: Ro, then the element tables, then the delay.
   std::vector<char> tabs;
   if (f) uni_tables<float>(type, gt, gr, Lt_, Lr_, tabs);
@@ -1762,13 +1850,16 @@ void DasUni::run_dev(const void *Y, double dt, double delay, double cp, int inte
     a.delay_rx = type_ == DAS_RCA_ROWTX;
   }
   a.y_stride = (type_ == DAS_SAFT ? Lt_ : Lt_ * Lr_) * K_;
+  require((uint64_t) a.y_stride * (prec_ == DREAM_HIP_F64 ? 8 : 4) < 0xFFFFFFF0ull,
+          "das_uni: Y must be smaller than 4 GiB");
   a.im_stride = No_;
   a.status = nullptr;
   h_->ensure_scratch_for(1);
   a.scratch = h_->scratch();
   a.scratch_bytes = h_->scratch_bytes();
   require(interp == DREAM_HIP_NEAREST || interp == DREAM_HIP_LINEAR, "das_uni: unknown interpolation");
-  launch_das_forward(a, type_, prec_, interp, false, s);
+  ChunkGate gate(s);
+  launch_das_forward(a, type_, prec_, interp, false, s, &gate);
 }
 
 void DasUni::run_host(const void *Y, double dt, double delay, double cp, void *Im)
@@ -1813,6 +1904,8 @@ DasBatch::DasBatch(const DreamDasProblem *probs, int32_t n, int type, int precis
     const bool saft = type == DAS_SAFT;
     require(p.K >= 1 && p.K < (1ll << 26), "das batch: A-scan length out of range (1 .. 2^26)");
     require(p.No >= 0 && p.Lt >= 1 && (saft || p.Lr >= 1), "das batch: bad array sizes");
+    require((type != DAS_RCA_COLTX && type != DAS_RCA_ROWTX) || rca_extents_in_bounds(p.Lt, p.Lr),
+            "das batch: RCA needs 2 Lt <= 3 Lr and Lr <= 3 Lt (stripe extents)");
     const int64_t n_ascans = saft ? p.Lt : p.Lt * p.Lr;
     require((uint64_t) (n_ascans * p.K) * es < 0xFFFFFFF0ull,
             "das batch: Y of one problem must be smaller than 4 GiB");

@@ -33,6 +33,9 @@ struct DasArgs {
   // handle's scratch for partial sums of split forward launches.
   const int32_t *perm;
   void *scratch;  size_t scratch_bytes;
+  // First block (forward tile, adjoint region or pixel-adjoint tile) and
+  // first frame of this launch: a long call runs as several chunk launches.
+  int64_t block0, frame0, block_end;
 };
 
 // Bytes of forward partial-sum scratch a launch of this shape uses.
@@ -48,7 +51,10 @@ void pixel_order(const GT *Ro, int64_t No, int32_t *perm);
 
 class DasHandle {
 public:
-  DasHandle(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, int precision);
+  // stripes_given: the caller supplies the RCA stripe extents (das_uni), so
+  // the element tables need not hold the entries the reference reads them from.
+  DasHandle(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, int precision,
+            bool stripes_given = false);
   ~DasHandle();
   bool changed(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, int precision) const;
 
@@ -159,9 +165,11 @@ private:
   hipGraphExec_t graph_ = nullptr;
 };
 
+// With a gate, a long launch runs as chunks with cancellation tests between
+// them (ChunkGate); without one it is a single launch.
 void launch_das_forward(const DasArgs &a, int type, int prec, int interp, bool apod,
-                        hipStream_t s);
+                        hipStream_t s, ChunkGate *gate = nullptr);
 void launch_das_adjoint(const DasArgs &a, int type, int prec, int interp, bool apod,
-                        bool want_gy, hipStream_t s);
+                        bool want_gy, hipStream_t s, ChunkGate *gate = nullptr);
 
 } // namespace dream

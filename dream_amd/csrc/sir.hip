@@ -248,14 +248,15 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
 
 template <typename T, bool ANY_GLOBAL>
 __global__ void __launch_bounds__(kThreads)
-sir_kernel(const SirDev *__restrict__ probs, int n_probs, int64_t n_items,
+sir_kernel(const SirDev *__restrict__ probs, int n_probs, int64_t item_lo, int64_t item_hi,
            unsigned long long *counter)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   double *hist = reinterpret_cast<double *>(smem);
+  // This launch serves items [item_lo, item_hi) (a chunk of a long plan).
   if (counter == nullptr) {
-    const int64_t item = blockIdx.x;
-    if (item >= n_items) return;
+    const int64_t item = item_lo + blockIdx.x;
+    if (item >= item_hi) return;
     const int p = find_problem(probs, n_probs, item);
     sir_item<T, ANY_GLOBAL>(probs[p], item - probs[p].item_begin, hist);
     return;
@@ -264,11 +265,11 @@ sir_kernel(const SirDev *__restrict__ probs, int n_probs, int64_t n_items,
   // resets both counters for the next launch.
   __shared__ int64_t s_item;
   for (;;) {
-    if (threadIdx.x == 0) s_item = (int64_t) atomicAdd(&counter[0], 1ull);
+    if (threadIdx.x == 0) s_item = item_lo + (int64_t) atomicAdd(&counter[0], 1ull);
     __syncthreads();
     const int64_t item = s_item;
     __syncthreads();
-    if (item >= n_items) break;
+    if (item >= item_hi) break;
     const int p = find_problem(probs, n_probs, item);
     sir_item<T, ANY_GLOBAL>(probs[p], item - probs[p].item_begin, hist);
   }
@@ -686,6 +687,8 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
   std::vector<double> tables;
   tables.reserve(table_doubles);
   host_.resize(P);
+  item_count_.assign(P, 0);
+  item_cost_.assign(P, 0.0);
   zero_len_.assign(P, 0);
   zero_ptr_.assign(P, nullptr);
   int64_t item = 0;
@@ -710,6 +713,8 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
     const int64_t items = (d.nx > 0 && d.ny > 0) ? d.No * d.chunks : 0;
     item += items;
     host_[k] = d;
+    item_count_[k] = items;
+    item_cost_[k] = cost[i];
     if (d.mode != SIR_LDS_STORE || items == 0) {
       zero_len_[i] = d.No * d.nt;
       zero_ptr_[i] = d.h;
@@ -759,14 +764,17 @@ void SirPlan::choose(hipStream_t s)
   std::snprintf(key, sizeof key, "sir|%d|%zu|%lld|%lld|%zu", out_dtype_, host_.size(),
                 (long long) n_items_, (long long) total_no, lds_bytes_);
   const int t = n_items_ ? autotune(key, (int) shapes_.size(),
-                                    [&](int i, hipStream_t st) { launch_shape(shapes_[(size_t) i], st); }, s)
+                                    [&](int i, hipStream_t st) { launch_shape(shapes_[(size_t) i], st, nullptr); }, s)
                          : -1;
   pick_ = t >= 0 ? t : 0;
 }
 
-void SirPlan::launch(hipStream_t s) { launch_shape(shapes_[(size_t) std::max(pick_, 0)], s); }
+void SirPlan::launch(hipStream_t s, ChunkGate *gate)
+{
+  launch_shape(shapes_[(size_t) std::max(pick_, 0)], s, gate);
+}
 
-void SirPlan::launch_shape(const SirShape &sh, hipStream_t s)
+void SirPlan::launch_shape(const SirShape &sh, hipStream_t s, ChunkGate *gate)
 {
   launch_fill_zero(d_status_, host_.size() * sizeof(Status), s);
   for (size_t i = 0; i < zero_len_.size(); i++)
@@ -778,22 +786,57 @@ void SirPlan::launch_shape(const SirShape &sh, hipStream_t s)
   unsigned long long *ctr = sh.persistent ? d_counter_ : nullptr;
   bool any_global = false;
   for (const SirDev &d : host_) any_global |= d.mode == SIR_GLOBAL;
-  if (out_dtype_ == DREAM_HIP_F64) {
-    if (any_global)
-      hipLaunchKernelGGL((sir_kernel<double, true>), dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
-                         d_probs_, (int) host_.size(), n_items_, ctr);
-    else
-      hipLaunchKernelGGL((sir_kernel<double, false>), dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
-                         d_probs_, (int) host_.size(), n_items_, ctr);
-  } else {
-    if (any_global)
-      hipLaunchKernelGGL((sir_kernel<float, true>), dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
-                         d_probs_, (int) host_.size(), n_items_, ctr);
-    else
-      hipLaunchKernelGGL((sir_kernel<float, false>), dim3(sh.grid), dim3(kThreads), lds_bytes_, s,
-                         d_probs_, (int) host_.size(), n_items_, ctr);
+  auto range = [&](int64_t lo, int64_t hi) {
+    const int64_t n = hi - lo;
+    const unsigned grid = (unsigned) (sh.persistent ? std::min<int64_t>(sh.grid, n) : n);
+    const int np = (int) host_.size();
+    if (out_dtype_ == DREAM_HIP_F64) {
+      if (any_global)
+        hipLaunchKernelGGL((sir_kernel<double, true>), dim3(grid), dim3(kThreads), lds_bytes_, s,
+                           d_probs_, np, lo, hi, ctr);
+      else
+        hipLaunchKernelGGL((sir_kernel<double, false>), dim3(grid), dim3(kThreads), lds_bytes_,
+                           s, d_probs_, np, lo, hi, ctr);
+    } else {
+      if (any_global)
+        hipLaunchKernelGGL((sir_kernel<float, true>), dim3(grid), dim3(kThreads), lds_bytes_, s,
+                           d_probs_, np, lo, hi, ctr);
+      else
+        hipLaunchKernelGGL((sir_kernel<float, false>), dim3(grid), dim3(kThreads), lds_bytes_, s,
+                           d_probs_, np, lo, hi, ctr);
+    }
+    DREAM_HIP_CHECK(hipGetLastError());
+  };
+  if (!gate) {
+    range(0, n_items_);
+    return;
   }
-  DREAM_HIP_CHECK(hipGetLastError());
+  // Chunks of about chunk_work() contributions: items are in the plan's
+  // order (largest first) and all items of one problem cost the same.
+  const double W = (double) chunk_work();
+  int64_t lo = 0, item = 0;
+  double acc = 0.0;
+  for (size_t k = 0; k < host_.size(); k++) {
+    const int64_t items = item_count_[k];
+    const double c = std::max(item_cost_[k], 1.0);
+    int64_t left = items;
+    while (left > 0) {
+      const int64_t take = std::min<int64_t>(left, (int64_t) std::ceil((W - acc) / c));
+      acc += (double) take * c;
+      item += take;
+      left -= take;
+      if (acc >= W && item < n_items_) {
+        gate->before();
+        range(lo, item);
+        gate->after();
+        lo = item;
+        acc = 0.0;
+      }
+    }
+  }
+  gate->before();
+  range(lo, n_items_);
+  gate->after();
 }
 
 void SirPlan::run(hipStream_t s, bool use_graph)
@@ -803,7 +846,8 @@ void SirPlan::run(hipStream_t s, bool use_graph)
   ran_ = true;
   choose(s);
   if (!use_graph) {
-    launch(s);
+    ChunkGate gate(s);
+    launch(s, &gate);
     return;
   }
   if (!graph_) {
@@ -812,7 +856,7 @@ void SirPlan::run(hipStream_t s, bool use_graph)
     hipStream_t cs = lib_stream();
     hipGraph_t g = nullptr;
     DREAM_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    launch(cs);
+    launch(cs, nullptr);
     DREAM_HIP_CHECK(hipStreamEndCapture(cs, &g));
     DREAM_HIP_CHECK(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
     DREAM_HIP_CHECK(hipGraphDestroy(g));

@@ -97,12 +97,16 @@ private:
     int grid;
   };
   void choose(hipStream_t s);               // launch shape: autotuned or heuristic
-  void launch(hipStream_t s);
-  void launch_shape(const SirShape &sh, hipStream_t s);
+  // With a gate, chunks of about chunk_work() contributions with
+  // cancellation tests between them; without, one launch.
+  void launch(hipStream_t s, ChunkGate *gate);
+  void launch_shape(const SirShape &sh, hipStream_t s, ChunkGate *gate);
   std::vector<SirShape> shapes_;            // candidates, heuristic first
   int pick_ = -1;
   int out_dtype_;
-  std::vector<SirDev> host_;
+  std::vector<SirDev> host_;                // problems in plan order
+  std::vector<int64_t> item_count_;         // per problem (plan order): work items
+  std::vector<double> item_cost_;           //   and contributions per item
   std::vector<int> levels_;
   std::vector<int64_t> zero_len_;  // per problem: elements to zero first (0 = none)
   std::vector<void *> zero_ptr_;

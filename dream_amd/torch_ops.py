@@ -36,6 +36,35 @@ def _dev_tensor(x, dtype, device):
     return t.contiguous()
 
 
+def _same_device(a, b):
+    a, b = torch.device(a), torch.device(b)
+    if a.type != b.type:
+        return False
+    ia = a.index if a.index is not None else torch.cuda.current_device()
+    ib = b.index if b.index is not None else torch.cuda.current_device()
+    return ia == ib
+
+
+def _check(t, name, device, dtype=None, numel=None, optional=False):
+    """Refuse, before any launch, a tensor the kernels cannot read as the
+    plan expects: host memory, another GPU, another dtype, a strided view or
+    too few elements would be read as raw device bytes."""
+    if t is None:
+        if optional:
+            return
+        raise ValueError(f"{name}: a tensor is required")
+    if not torch.is_tensor(t):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.device.type != "cuda" or not _same_device(t.device, device):
+        raise ValueError(f"{name}: must be on {device}, is on {t.device}")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"{name}: must be {dtype}, is {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name}: must have {numel} elements, has {t.numel()}")
+
+
 def pixel_order_of(Ro):
     """Spatially compact visiting order of the points Ro ((No, 3))."""
     R = np.asfortranarray(np.asarray(Ro.cpu() if torch.is_tensor(Ro) else Ro,
@@ -102,6 +131,8 @@ class DasPlan:
 
     def _delay(self, delay):
         if torch.is_tensor(delay) and delay.numel() > 1:
+            if delay.numel() != self.No:
+                raise ValueError(f"delay: needs 1 or {self.No} elements, has {delay.numel()}")
             d = delay.to(device=self.device, dtype=self.geom_dtype).contiguous()
             return d, 1
         val = float(delay)
@@ -110,18 +141,27 @@ class DasPlan:
         return self._delay_cache[val], 0
 
     def _frames(self, Y):
-        if Y.dim() == 2:
-            return 1
-        assert Y.dim() == 3, "Y must be (n_ascans, K) or (F, n_ascans, K)"
-        return Y.shape[0]
+        _check(Y, "Y", self.device, self.data_dtype)
+        if Y.dim() not in (2, 3) or tuple(Y.shape[-2:]) != (self.n_ascans, self.K):
+            raise ValueError(f"Y must be ({self.n_ascans}, {self.K}) or "
+                             f"(F, {self.n_ascans}, {self.K}), is {tuple(Y.shape)}")
+        return 1 if Y.dim() == 2 else Y.shape[0]
+
+    def _apod(self, apod_t, apod_r):
+        _check(apod_t, "apod_t", self.device, self.data_dtype, self.Lt, optional=True)
+        if self.type != 0:
+            _check(apod_r, "apod_r", self.device, self.data_dtype, self.Lr, optional=True)
+            if (apod_t is None) != (apod_r is None):
+                raise ValueError("TFM/RCA apodization needs both apod_t and apod_r")
 
     def forward(self, Y, dt, delay, cp, apod_t=None, apod_r=None, interp="nearest",
                 err_level="ignore", want_status=False, out=None):
         """Beamform; returns Im (and the SIRError code when want_status)."""
-        assert Y.is_contiguous() and Y.dtype == self.data_dtype and Y.shape[-2:] == (
-            self.n_ascans, self.K), "Y must be contiguous (.., n_ascans, K) of the plan's dtype"
         F = self._frames(Y)
+        self._apod(apod_t, apod_r)
         d, dtype = self._delay(delay)
+        if out is not None:
+            _check(out, "out", self.device, self.data_dtype, F * self.No)
         Im = out if out is not None else torch.empty(
             (F, self.No) if Y.dim() == 3 else (self.No,), dtype=self.data_dtype, device=Y.device)
         code = ctypes.c_int(0)
@@ -136,7 +176,9 @@ class DasPlan:
                 want_gy=True, want_apod=True, want_cp=True):
         """Aᵀg: returns (gY or None, g_apod_t, g_apod_r, g_cp)."""
         F = self._frames(Y)
-        g = g.contiguous()
+        self._apod(apod_t, apod_r)
+        g = g.contiguous() if torch.is_tensor(g) else g
+        _check(g, "g", self.device, self.data_dtype, F * self.No)
         dt_ = self.data_dtype
         gY = torch.empty_like(Y) if want_gy else None
         gat = torch.empty(self.Lt, dtype=dt_, device=Y.device) if want_apod else None
@@ -194,6 +236,40 @@ def _host_f64(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
 
 
+def _check_points(Ro):
+    """Ro: (3, No) contiguous float64 tensor on a GPU; returns No."""
+    if not torch.is_tensor(Ro) or Ro.device.type != "cuda":
+        raise ValueError("Ro must be a (3, No) float64 tensor on the GPU")
+    _check(Ro, "Ro", Ro.device, torch.float64)
+    if Ro.dim() != 2 or Ro.shape[0] != 3:
+        raise ValueError(f"Ro must be (3, No), is {tuple(Ro.shape)}")
+    return Ro.shape[1]
+
+
+def _sir_delay(delay, Ro, No):
+    """A scalar delay, or one per point moved to Ro's device as float64."""
+    if torch.is_tensor(delay) and delay.numel() > 1:
+        if delay.numel() != No:
+            raise ValueError(f"delay: needs 1 or {No} elements, has {delay.numel()}")
+        return delay.to(device=Ro.device, dtype=torch.float64).contiguous(), 1
+    return torch.tensor([float(delay)], dtype=torch.float64, device=Ro.device), 0
+
+
+def _host_lengths(foc_met, focal_v, apod_met, apod_v, L):
+    """'ud' focusing and apodization read one host value per element."""
+    if foc_met == "ud" and focal_v.size < L:
+        raise ValueError(f"focal: 'ud' focusing needs {L} delays, has {focal_v.size}")
+    if apod_met == "ud" and apod_v.size < L:
+        raise ValueError(f"apod: 'ud' apodization needs {L} weights, has {apod_v.size}")
+
+
+def _sir_out(out, No, nt, out_dtype, device):
+    if out is None:
+        return torch.empty((No, int(nt)), dtype=out_dtype, device=device)
+    _check(out, "out", device, out_dtype, No * int(nt))
+    return out
+
+
 def arr_rect(Ro, a, b, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=0.0,
              steer_met="off", theta=0.0, phi=0.0, apod_met="off", apod=None, apod_par=0.0,
              err_level="ignore", out_dtype=torch.float64, out=None):
@@ -201,19 +277,16 @@ def arr_rect(Ro, a, b, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=0.0
     delay: float or (No,) device tensor.  Returns (H (No, nt), SIRError code,
     out-of-bounds contribution count)."""
     _lib.require_device()
-    assert Ro.dtype == torch.float64 and Ro.is_contiguous() and Ro.shape[0] == 3
-    No = Ro.shape[1]
+    No = _check_points(Ro)
     G = np.asarray(G, dtype=np.float64).reshape(-1, 3)
     L = G.shape[0]
     Gf = np.asfortranarray(G)
-    if torch.is_tensor(delay) and delay.numel() > 1:
-        d, dtype = delay.to(dtype=torch.float64).contiguous(), 1
-    else:
-        d, dtype = torch.tensor([float(delay)], dtype=torch.float64, device=Ro.device), 0
+    d, dtype = _sir_delay(delay, Ro, No)
     focal_v = _host_f64(np.atleast_1d(focal))
     apod_v = _host_f64(np.ones(L) if apod is None else apod)
+    _host_lengths(foc_met, focal_v, apod_met, apod_v, L)
     do_apod = 0 if apod_met == "off" else 1
-    H = out if out is not None else torch.empty((No, int(nt)), dtype=out_dtype, device=Ro.device)
+    H = _sir_out(out, No, nt, out_dtype, Ro.device)
     code = ctypes.c_int(0)
     oob = ctypes.c_int64(0)
     _lib.check(_lib.lib().dream_hip_arr_rect_dev(
@@ -232,17 +305,14 @@ def arr_circ(Ro, R, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", focal=0.0,
     conventions of arr_rect.  Returns (H (No, nt), SIRError code, out-of-bounds
     contribution count)."""
     _lib.require_device()
-    assert Ro.dtype == torch.float64 and Ro.is_contiguous() and Ro.shape[0] == 3
-    No = Ro.shape[1]
+    No = _check_points(Ro)
     Gf = np.asfortranarray(np.asarray(G, dtype=np.float64).reshape(-1, 3))
     L = Gf.shape[0]
-    if torch.is_tensor(delay) and delay.numel() > 1:
-        d, dtype = delay.to(dtype=torch.float64).contiguous(), 1
-    else:
-        d, dtype = torch.tensor([float(delay)], dtype=torch.float64, device=Ro.device), 0
+    d, dtype = _sir_delay(delay, Ro, No)
     focal_v = _host_f64(np.atleast_1d(focal))
     apod_v = _host_f64(np.ones(L) if apod is None else apod)
-    H = out if out is not None else torch.empty((No, int(nt)), dtype=out_dtype, device=Ro.device)
+    _host_lengths(foc_met, focal_v, apod_met, apod_v, L)
+    H = _sir_out(out, No, nt, out_dtype, Ro.device)
     code = ctypes.c_int(0)
     oob = ctypes.c_int64(0)
     _lib.check(_lib.lib().dream_hip_arr_circ_dev(
@@ -261,17 +331,14 @@ def arr_cylind(Ro, a, b, Rcurv, dx, dy, dt, nt, delay, v, cp, G, foc_met="off", 
     curvature radius Rcurv); the conventions of arr_rect.  Returns (H (No, nt),
     SIRError code, out-of-bounds contribution count)."""
     _lib.require_device()
-    assert Ro.dtype == torch.float64 and Ro.is_contiguous() and Ro.shape[0] == 3
-    No = Ro.shape[1]
+    No = _check_points(Ro)
     Gf = np.asfortranarray(np.asarray(G, dtype=np.float64).reshape(-1, 3))
     L = Gf.shape[0]
-    if torch.is_tensor(delay) and delay.numel() > 1:
-        d, dtype = delay.to(dtype=torch.float64).contiguous(), 1
-    else:
-        d, dtype = torch.tensor([float(delay)], dtype=torch.float64, device=Ro.device), 0
+    d, dtype = _sir_delay(delay, Ro, No)
     focal_v = _host_f64(np.atleast_1d(focal))
     apod_v = _host_f64(np.ones(L) if apod is None else apod)
-    H = out if out is not None else torch.empty((No, int(nt)), dtype=out_dtype, device=Ro.device)
+    _host_lengths(foc_met, focal_v, apod_met, apod_v, L)
+    H = _sir_out(out, No, nt, out_dtype, Ro.device)
     code = ctypes.c_int(0)
     oob = ctypes.c_int64(0)
     _lib.check(_lib.lib().dream_hip_arr_cylind_dev(
@@ -290,19 +357,15 @@ def arr_annu(Ro, Gr, dx, dy, dt, nt, delay, v, cp, foc_met="off", focal=0.0,
     vector of radii (centre disc, then inner/outer per ring).  Returns
     (H (No, nt), SIRError code, out-of-bounds contribution count)."""
     _lib.require_device()
-    assert Ro.dtype == torch.float64 and Ro.is_contiguous() and Ro.shape[0] == 3
-    No = Ro.shape[1]
+    No = _check_points(Ro)
     Gr_v = _host_f64(np.asarray(Gr, dtype=np.float64).ravel())
     nr = Gr_v.shape[0]
-    if torch.is_tensor(delay) and delay.numel() > 1:
-        d, dtype = delay.to(dtype=torch.float64).contiguous(), 1
-    else:
-        d, dtype = torch.tensor([float(delay)], dtype=torch.float64, device=Ro.device), 0
+    d, dtype = _sir_delay(delay, Ro, No)
     focal_v = _host_f64(np.atleast_1d(focal))
     if focal_v.shape[0] < nr:
         focal_v = _host_f64(np.resize(focal_v, nr))
     apod_v = _host_f64(np.ones(nr) if apod is None else apod)
-    H = out if out is not None else torch.empty((No, int(nt)), dtype=out_dtype, device=Ro.device)
+    H = _sir_out(out, No, nt, out_dtype, Ro.device)
     code = ctypes.c_int(0)
     oob = ctypes.c_int64(0)
     _lib.check(_lib.lib().dream_hip_arr_annu_dev(
@@ -335,17 +398,13 @@ class SirBatch:
         arr = (_lib.DreamSirProblem * max(self.n, 1))()
         for i, p in enumerate(problems):
             Ro = p["Ro"]
-            assert Ro.dtype == torch.float64 and Ro.is_contiguous() and Ro.shape[0] == 3
-            No = Ro.shape[1]
+            No = _check_points(Ro)
             G = np.asfortranarray(np.asarray(p["G"], dtype=np.float64).reshape(-1, 3))
             L = G.shape[0]
-            delay = p.get("delay", 0.0)
-            if torch.is_tensor(delay) and delay.numel() > 1:
-                d, dtype = delay.to(dtype=torch.float64).contiguous(), 1
-            else:
-                d, dtype = torch.tensor([float(delay)], dtype=torch.float64, device=Ro.device), 0
+            d, dtype = _sir_delay(p.get("delay", 0.0), Ro, No)
             focal = _host_f64(np.atleast_1d(p.get("focal", 0.0)))
             apod = _host_f64(p["apod"] if p.get("apod") is not None else np.ones(L))
+            _host_lengths(p.get("foc_met", "off"), focal, p.get("apod_met", "off"), apod, L)
             H = torch.zeros((No, int(p["nt"])), dtype=out_dtype, device=Ro.device)
             self.H.append(H)
             self._keep += [Ro, G, d, focal, apod, H]
@@ -421,8 +480,12 @@ class DasBatch:
             No, Lt = Ro.shape[1], Gt.shape[1]
             Lr = 0 if Gr is None else Gr.shape[1]
             n_ascans = Lt if self.type == 0 else Lt * Lr
-            assert Y.is_contiguous() and Y.dtype == self.data_dtype and \
-                tuple(Y.shape) == (n_ascans, int(p["K"])), "Y must be contiguous (n_ascans, K)"
+            _check(Y, f"problem {i}: Y", dev, self.data_dtype, n_ascans * int(p["K"]))
+            _check(p.get("apod_t"), f"problem {i}: apod_t", dev, self.data_dtype, Lt,
+                   optional=True)
+            if self.type != 0:
+                _check(p.get("apod_r"), f"problem {i}: apod_r", dev, self.data_dtype, Lr,
+                       optional=True)
             perm = None
             if pixel_order:
                 perm = torch.as_tensor(pixel_order_of(p["Ro"]), device=dev)
@@ -469,22 +532,33 @@ def conv_sum(A, B, out=None, mode="="):
     updates out (N, M_A + M_B - 1): out[n] (mode) sum_l A[l, n] (*) B[...].
     float64 or float32.
     """
+    if not torch.is_tensor(A) or A.dtype not in (torch.float64, torch.float32):
+        raise ValueError("A must be a float64 or float32 tensor")
+    _check(A, "A", A.device, A.dtype)
+    _check(B, "B", A.device, A.dtype)
+    if out is not None:
+        _check(out, "out", A.device, A.dtype)
     A3 = A if A.dim() == 3 else A.unsqueeze(0)
     L, N, MA = A3.shape
-    assert A3.is_contiguous() and B.is_contiguous() and B.dtype == A.dtype
     MB = B.shape[-1]
     if B.dim() == 1:
         b_col, b_layer = 0, 0
-    elif A.dim() == 2:
-        assert B.shape[0] == N, "B needs one row per column of A"
+    elif A.dim() == 2 and B.dim() == 2:
+        if B.shape[0] != N:
+            raise ValueError("B needs one row per column of A")
         b_col, b_layer = MB, 0
-    else:
-        assert B.shape[0] == L, "B needs one row per layer of A"
+    elif B.dim() == 2:
+        if B.shape[0] != L:
+            raise ValueError("B needs one row per layer of A")
         b_col, b_layer = 0, MB
+    else:
+        raise ValueError("B must be (M_B,), (N, M_B) or (L, M_B)")
     if out is None:
-        assert mode == "=", "in-place modes need `out`"
+        if mode != "=":
+            raise ValueError("in-place modes need `out`")
         out = torch.empty((N, MA + MB - 1), dtype=A.dtype, device=A.device)
-    assert out.is_contiguous() and out.shape == (N, MA + MB - 1) and out.dtype == A.dtype
+    if tuple(out.shape) != (N, MA + MB - 1):
+        raise ValueError(f"out must be ({N}, {MA + MB - 1}), is {tuple(out.shape)}")
     dt = _lib.F64 if A.dtype == torch.float64 else _lib.F32
     _lib.check(_lib.lib().dream_hip_conv_sum_dev(_ptr(out), _ptr(A3), MA, N, L, MA, N * MA,
                                                  _ptr(B), MB, b_col, b_layer, CONV_MODE[mode],

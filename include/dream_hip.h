@@ -84,13 +84,31 @@ int dream_hip_device_count(void);
  * reference. */
 void dream_hip_set_autotune(int on);
 
-/* Cooperative cancellation: dream_hip_abort() makes running calls stop
- * between work chunks and return DREAM_HIP_ECANCELLED; it is safe to call
- * from a signal handler.  Mirrors ArrRect::abort / DAS<T>::abort /
+/* Cooperative cancellation: dream_hip_abort() clears the running flag; it is
+ * safe to call from a signal handler or another thread.  Every entry point
+ * tests the flag on entry.  Long eager calls (DAS forwards and adjoints, SIR
+ * plans and the host-pointer SIR entry points) run as consecutive chunk
+ * launches of about 4 ms of device time each and test the flag before each
+ * chunk, keeping at most two chunks queued ahead of the device (the calling
+ * thread waits for the device meanwhile), so a call cancelled while it runs
+ * stops within about two chunks and returns DREAM_HIP_ECANCELLED with the
+ * outputs of the launched chunks written.  Calls that capture into or replay
+ * a hipGraph (SIR plans and DAS batches with use_graph), DAS batches and the
+ * attenuated SIR test the flag on entry only.  The reference's threads test
+ * `running` after every observation point (src/dream_arr_rect.cc:177-187,
+ * extra_src/das.cc:180-186).  Mirrors ArrRect::abort / DAS<T>::abort /
  * set_running / is_running (include/dream_arr_rect.h:53-54, include/das.h:74-76). */
 void dream_hip_abort(int signum);
 void dream_hip_set_running(void);
 int dream_hip_is_running(void);
+
+/* Test hook, no counterpart in the reference: work units (pixel-channel
+ * pairs or sub-element contributions) per chunk of a long call (<= 0: the
+ * default, 10^10), and, when cancel_after_chunks > 0, a dream_hip_abort()
+ * issued by the library itself right after that many further chunks have
+ * been launched (<= 0: off).  Lets tests run small problems as many chunks
+ * and cancel a call deterministically between two of them. */
+void dream_hip_test_chunking(int64_t work_per_chunk, int64_t cancel_after_chunks);
 
 /* ---------------------------------------------------------------- SIR */
 

@@ -27,9 +27,14 @@
 #include "dream_hip.h"
 
 namespace dream_hip_detail {
-inline void check(int rc)
+// false when dream_hip_abort() stopped the call: the running flag stays
+// cleared, so the gateway's own `if (!obj.is_running())` branch reports the
+// interruption as it does for the reference's threads.
+inline bool check(int rc)
 {
+  if (rc == DREAM_HIP_ECANCELLED) return false;
   if (rc != DREAM_HIP_OK) throw std::runtime_error(std::string("dream_hip: ") + dream_hip_last_error());
+  return true;
 }
 inline SIRError finish(int code)
 {
@@ -54,10 +59,11 @@ public:
   {
     dream_hip_set_running();
     int code = 0;
-    dream_hip_detail::check(dream_hip_arr_rect(
-        alpha, Ro, (int64_t) No, a, b, dx, dy, dt, (int64_t) nt, (int) delay_type, delay, v, cp,
-        (int64_t) num_elements, G, (int) foc_met, focal, (int) steer_met, theta, phi, apod,
-        do_apod ? 1 : 0, (int) apod_met, apod_par, h, (int) err_level, &code));
+    if (!dream_hip_detail::check(dream_hip_arr_rect(
+            alpha, Ro, (int64_t) No, a, b, dx, dy, dt, (int64_t) nt, (int) delay_type, delay, v,
+            cp, (int64_t) num_elements, G, (int) foc_met, focal, (int) steer_met, theta, phi,
+            apod, do_apod ? 1 : 0, (int) apod_met, apod_par, h, (int) err_level, &code)))
+      return SIRError::none;
     return dream_hip_detail::finish(code);
   }
 
@@ -77,9 +83,10 @@ public:
   {
     dream_hip_set_running();
     int code = 0;
-    dream_hip_detail::check(dream_hip_rect(alpha, Ro, (int64_t) No, a, b, dx, dy, dt,
-                                           (int64_t) nt, (int) delay_type, delay, v, cp, h,
-                                           (int) err_level, &code));
+    if (!dream_hip_detail::check(dream_hip_rect(alpha, Ro, (int64_t) No, a, b, dx, dy, dt,
+                                                (int64_t) nt, (int) delay_type, delay, v, cp, h,
+                                                (int) err_level, &code)))
+      return SIRError::none;
     return dream_hip_detail::finish(code);
   }
 
@@ -125,8 +132,10 @@ public:
                const T *delay, T cp, T *Im, ErrorLevel err_level)
   {
     int code = 0;
-    dream_hip_detail::check(dream_hip_das(m_h, Y, Ro, gt, gr, (double) dt, (int) delay_type,
-                                          delay, (double) cp, Im, (int) err_level, &code));
+    if (!dream_hip_detail::check(dream_hip_das(m_h, Y, Ro, gt, gr, (double) dt,
+                                               (int) delay_type, delay, (double) cp, Im,
+                                               (int) err_level, &code)))
+      return SIRError::none;
     return dream_hip_detail::finish(code);
   }
 

@@ -898,14 +898,14 @@ static int das_point(int type, const T *Y, idx_t K, const T *Gt, idx_t Lt, const
     }
   } else {
     // RCA: Gt = column electrodes (along x), Gr = row electrodes (along y).
-    // The stripe extents are the first/last y of the rows and the first/last
-    // x of the columns.  The reference indexes them with the other array's
-    // length (:464-467, :560-563); the two agree for square arrays, the
-    // only case the reference's callers use.
+    // The stripe extents are read as the reference reads them (:464-467,
+    // :560-563): rows' y at offsets Lc and 2 Lc - 1, columns' x at 0 and
+    // Lr - 1 -- the first/last y of the rows and x of the columns for square
+    // arrays; for non-square ones the same (other) entries the reference uses.
     const T *Gc = Gt, *Gw = Gr;
     const idx_t Lc = Lt, Lw = Lr;
-    const T y_lo = Gw[0 + Lw], y_hi = Gw[Lw - 1 + Lw];
-    const T x_lo = Gc[0], x_hi = Gc[Lc - 1];
+    const T y_lo = Gw[0 + Lc], y_hi = Gw[Lc - 1 + Lc];
+    const T x_lo = Gc[0], x_hi = Gc[Lw - 1];
     const T gy_c = stripe_gap<T>(yo, y_lo, y_hi);
     const T gx_r = stripe_gap<T>(xo, x_lo, x_hi);
     if (type == DAS_RCA_COLTX) {                            // :437-529
@@ -1010,8 +1010,8 @@ static void legs(const Geo &g, double xo, double yo, double zo, double *dtx, dou
         drx[j] = dist3<double>(g.Gr[j] - xo, g.Gr[j + g.Lr] - yo, g.Gr[j + 2 * g.Lr] - zo);
     return;
   }
-  const double y_lo = g.Gr[g.Lr], y_hi = g.Gr[g.Lr - 1 + g.Lr];
-  const double x_lo = g.Gt[0], x_hi = g.Gt[g.Lt - 1];
+  const double y_lo = g.Gr[g.Lt], y_hi = g.Gr[g.Lt - 1 + g.Lt];
+  const double x_lo = g.Gt[0], x_hi = g.Gt[g.Lr - 1];
   const double gy_c = stripe_gap<double>(yo, y_lo, y_hi);
   const double gx_r = stripe_gap<double>(xo, x_lo, x_hi);
   for (idx_t i = 0; i < g.Lt; i++)

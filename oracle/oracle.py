@@ -379,6 +379,14 @@ def _das_args(Y, Gt, Gr, Ro, delay, conv):
     return Y, Gt, Gr, Ro, delay
 
 
+def _rca_shapes_ok(method, Lt, Lr):
+    """The reference reads the RCA stripe extents at rows[Lt], rows[2 Lt - 1]
+    and columns[Lr - 1] (extra_src/das.cc:464-467); outside 2 Lt <= 3 Lr and
+    Lr <= 3 Lt those reads leave the element tables (undefined behaviour)."""
+    if DAS_TYPE[method] >= 2 and Lr > 0 and not (2 * Lt <= 3 * Lr and Lr <= 3 * Lt):
+        raise ValueError("RCA stripe extents out of the element tables")
+
+
 def das(Y, Gt, Gr, Ro, dt, delay, cp, method="tfm", err_level="ignore", precision="f64",
         nthreads=0, use_ref=False):
     """DAS<T>::das.  precision 'f64' or 'f32' selects T.  Returns (Im, err code).
@@ -393,6 +401,7 @@ def das(Y, Gt, Gr, Ro, dt, delay, cp, method="tfm", err_level="ignore", precisio
     Lt = Gt.shape[0]
     Lr = 0 if Gr is None or DAS_TYPE[method] == 0 else Gr.shape[0]
     No = Ro.shape[0]
+    _rca_shapes_ok(method, Lt, Lr)
     dtype = 0 if delay.size == 1 else 1
     Im = np.zeros(No, dtype=np.float32 if f32 else np.float64)
     gr_p = _p(Gr, ptr) if Gr is not None else None
@@ -417,6 +426,7 @@ def das_ext(Y, Gt, Gr, Ro, dt, delay, cp, method="saft", interp="linear", apod_t
     Y, Gt, Gr, Ro, _ = _das_args(Y, Gt, Gr, Ro, [0.0], _f64)
     K, Lt, No = Y.shape[0], Gt.shape[0], Ro.shape[0]
     Lr = 0 if Gr is None or DAS_TYPE[method] == 0 else Gr.shape[0]
+    _rca_shapes_ok(method, Lt, Lr)
     at = None if apod_t is None else _f64(np.asarray(apod_t).ravel())
     ar = None if apod_r is None else _f64(np.asarray(apod_r).ravel())
     Im = np.zeros(No, dtype=np.float64)
@@ -432,6 +442,7 @@ def das_ext_adjoint(Y, Gt, Gr, Ro, dt, delay, cp, g, method="saft", interp="line
     Y, Gt, Gr, Ro, _ = _das_args(Y, Gt, Gr, Ro, [0.0], _f64)
     K, Lt, No = Y.shape[0], Gt.shape[0], Ro.shape[0]
     Lr = 0 if Gr is None or DAS_TYPE[method] == 0 else Gr.shape[0]
+    _rca_shapes_ok(method, Lt, Lr)
     at = None if apod_t is None else _f64(np.asarray(apod_t).ravel())
     ar = None if apod_r is None else _f64(np.asarray(apod_r).ravel())
     g = _f64(np.asarray(g).ravel())

@@ -7,6 +7,8 @@ small seeded inputs and stores inputs and outputs as .npz fixtures:
   das_ref.npz             DAS<double>/DAS<float>::das for saft, tfm,
                           rca_coltx, rca_rowtx; scalar and per-pixel delays;
                           ignore / warn error levels (outputs and codes)
+  das_rca_nonsquare_ref.npz  both RCA methods on arrays of 8 x 11 and 12 x 9
+                          elements (the stripe extents' indexing)
   arr_functions_ref.npz   max_dim_arr, focusing (all methods), beamsteering
                           (all methods), apodization (all methods)
 
@@ -75,6 +77,34 @@ def make_das():
     return len(out)
 
 
+def make_das_rca_nonsquare():
+    """RCA arrays with different column and row counts: the reference reads
+    the stripe extents at rows[Lc], rows[2 Lc - 1] and columns[Lr - 1]
+    (extra_src/das.cc:464-467, :560-563), so these vectors fix which entries
+    it uses."""
+    rng = np.random.default_rng(SEED + 2)
+    out = {}
+    K = 400
+    for Lc, Lr in ((8, 11), (12, 9)):
+        xc = np.sort(rng.uniform(-3.0, 3.0, Lc))
+        yr = np.sort(rng.uniform(-3.0, 3.0, Lr))
+        G_col = np.c_[xc, rng.uniform(-0.2, 0.2, Lc), rng.uniform(0.0, 0.1, Lc)]
+        G_row = np.c_[rng.uniform(-0.2, 0.2, Lr), yr, rng.uniform(0.0, 0.1, Lr)]
+        Ro = np.c_[rng.uniform(-5, 5, 160), rng.uniform(-5, 5, 160), rng.uniform(0.5, 6, 160)]
+        Y = rng.standard_normal((K, Lc * Lr))
+        key = f"c{Lc}r{Lr}"
+        out[f"{key}__Y"], out[f"{key}__Gt"], out[f"{key}__Gr"] = Y, G_col, G_row
+        out[f"{key}__Ro"] = Ro
+        for method in ("rca_coltx", "rca_rowtx"):
+            for prec in ("f64", "f32"):
+                Im, err = O.das(Y, G_col, G_row, Ro, 0.025, 0.3, 1510.0, method, "warn", prec,
+                                use_ref=True)
+                out[f"{key}__{method}__{prec}__Im"] = Im
+                out[f"{key}__{method}__{prec}__err"] = np.int32(err)
+    np.savez_compressed(os.path.join(HERE, "das_rca_nonsquare_ref.npz"), **out)
+    return len(out)
+
+
 def make_arr_functions():
     import ctypes
     R = O.ref()
@@ -108,5 +138,6 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref/libdream_ref.so is missing: run `make -C oracle ref` first")
     os.environ.setdefault("DREAM_NUM_THREADS", "4")
     print("das_ref.npz entries:", make_das())
+    print("das_rca_nonsquare_ref.npz entries:", make_das_rca_nonsquare())
     make_arr_functions()
     print("arr_functions_ref.npz written")

@@ -68,3 +68,37 @@ def test_oracle_matches_reference_random(oracle, method):
             b, eb = oracle.das(Y, Gt, Gr, Ro, 0.02, delay, 1540.0, method, "warn", prec,
                                use_ref=True)
             assert np.array_equal(a, b) and ea == eb
+
+
+def test_oracle_matches_reference_nonsquare_rca(oracle):
+    """Non-square row-column arrays: the reference reads the stripe extents at
+    rows[Lc], rows[2 Lc - 1] and columns[Lr - 1] (extra_src/das.cc:464-467,
+    :560-563); the oracle must reproduce its vectors bit for bit."""
+    d = np.load(os.path.join(HERE, "golden", "das_rca_nonsquare_ref.npz"))
+    for key in ("c8r11", "c12r9"):
+        Y, Gt, Gr, Ro = (d[f"{key}__{n}"] for n in ("Y", "Gt", "Gr", "Ro"))
+        for method in ("rca_coltx", "rca_rowtx"):
+            for prec in ("f64", "f32"):
+                Im, err = oracle.das(Y, Gt, Gr, Ro, 0.025, 0.3, 1510.0, method, "warn", prec,
+                                     nthreads=3)
+                assert np.array_equal(Im, d[f"{key}__{method}__{prec}__Im"]), (key, method, prec)
+                assert err == int(d[f"{key}__{method}__{prec}__err"])
+    # The vectors are sensitive to that indexing: with 8 columns and 11 rows
+    # the lower y extent is rows[8], the x of row 8, which no leg reads
+    # (rows' legs use the stripe gap, y and z), so moving the rows' x
+    # coordinates changes the reference's image only through the extent.
+    Y, Gt, Gr, Ro = (d[f"c8r11__{n}"] for n in ("Y", "Gt", "Gr", "Ro"))
+    Gr2 = Gr.copy()
+    Gr2[:, 0] += 1.5
+    Im2, _ = oracle.das(Y, Gt, Gr2, Ro, 0.025, 0.3, 1510.0, "rca_coltx", "warn", "f64",
+                        nthreads=3)
+    assert not np.array_equal(Im2, d["c8r11__rca_coltx__f64__Im"])
+
+
+def test_oracle_refuses_rca_extents_outside_the_tables(oracle):
+    rng = np.random.default_rng(1)
+    Y = rng.standard_normal((50, 4 * 20))
+    Gt = np.c_[np.linspace(-1, 1, 4), np.zeros(4), np.zeros(4)]
+    Gr = np.c_[np.zeros(20), np.linspace(-1, 1, 20), np.zeros(20)]
+    with pytest.raises(ValueError):
+        oracle.das(Y, Gt, Gr, np.array([[0.0, 0.0, 1.0]]), 0.02, 0.0, 1500.0, "rca_coltx")
