@@ -648,6 +648,26 @@ int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const
   });
 }
 
+int dream_hip_das_adjoint_dev_noalloc(void *handle, const void *Y, const void *Ro,
+                                      const void *Gt, const void *Gr, double dt, double delay,
+                                      double cp, const void *apod_t, const void *apod_r,
+                                      int interp, int64_t n_frames, int64_t y_frame_stride,
+                                      int64_t im_frame_stride, const void *g, void *gY,
+                                      void *g_apod_t, void *g_apod_r, void *g_cp, void *stream)
+{
+  return guarded([&]() {
+    require(handle != nullptr, "das adjoint: null handle");
+    hipStream_t s = (hipStream_t) stream;   // NULL: the default stream
+    size_t bytes = 0;
+    double *z = fixed_workspace(&bytes);
+    static_cast<DasHandle *>(handle)->adjoint_dev(Y, Ro, Gt, Gr, dt, delay, cp, apod_t, apod_r,
+                                                  interp, n_frames, y_frame_stride,
+                                                  im_frame_stride, g, gY, g_apod_t, g_apod_r,
+                                                  g_cp, s, z, bytes);
+    return DREAM_HIP_OK;
+  });
+}
+
 // ------------------------------------------------------------------ das_uni
 
 int dream_hip_das_uni_create(int das_type, int64_t a_scan_len, const double *gt,

@@ -43,6 +43,11 @@ inline void require(bool ok, const std::string &msg)
 {
   if (!ok) throw Error(DREAM_HIP_EINVAL, msg);
 }
+// Literal messages: no string is built unless the check fails.
+inline void require(bool ok, const char *msg)
+{
+  if (!ok) throw Error(DREAM_HIP_EINVAL, msg);
+}
 
 // Device-side status block of one SIR problem or DAS call.
 struct Status {

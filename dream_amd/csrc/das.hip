@@ -29,6 +29,7 @@
 // weights and cp.
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -40,6 +41,13 @@
 
 #include "das.hpp"
 #include "tune.hpp"
+
+// The allocation-free adjoint's workspace (dream_hip_das_adjoint_dev_noalloc):
+// per-A-scan sums, per-element sums and the cp sum, FP64, 2 MiB.  Each GPU
+// the library runs on holds its own copy until the process ends; host code
+// uses only the per-device address fixed_workspace() looks up, never this
+// name.
+static __device__ __align__(256) double scratch[(size_t) 1 << 18];
 
 namespace dream {
 
@@ -1194,10 +1202,13 @@ void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t
       n_gj = (a.Lr + kGJ - 1) / kGJ;
       groups = (int64_t) ((a.Lt + kGI - 1) / kGI) * n_gj;
     }
-    std::vector<int> subs_cand{1};
-    for (int sb = 2; sb <= 4; sb *= 2)
+    // Largest first (the default), then smaller; a fixed array, so the
+    // launch takes no host memory.
+    int subs_cand[3], n_subs = 0;
+    for (int sb = 4; sb >= 2; sb /= 2)
       if ((regions1 + sb - 1) / sb * groups * a.n_frames >= 4 * (int64_t) cu_count())
-        subs_cand.insert(subs_cand.begin(), sb);
+        subs_cand[n_subs++] = sb;
+    subs_cand[n_subs++] = 1;
     const int na = TYPE == DAS_SAFT ? kSaftGroup : kGI * kGJ;
     const size_t lds = want_gy ? (size_t) na * kWin * sizeof(double) : 0;
     const int64_t n_z = TYPE == DAS_SAFT ? a.Lt : (int64_t) a.Lt * a.Lr;
@@ -1221,14 +1232,14 @@ void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t
                   });
     };
     int pick = 0;
-    if (subs_cand.size() > 1) {
+    if (n_subs > 1 && !autotune_suspended()) {
       char key[192];
       std::snprintf(key, sizeof key, "das_adj|%d|%d|%d|%d|%d|%d|%lld|%d|%d|%lld|%lld", TYPE,
                     (int) sizeof(DT), (int) sizeof(GT), INTERP, (int) APOD, (int) want_gy,
                     (long long) a.No, a.Lt, a.Lr, (long long) a.K, (long long) a.n_frames);
       const int before = autotune_cache_size();
-      const int t = autotune(key, (int) subs_cand.size(),
-                             [&](int i, hipStream_t st) { run(subs_cand[(size_t) i], true, st, nullptr); },
+      const int t = autotune(key, n_subs,
+                             [&](int i, hipStream_t st) { run(subs_cand[i], true, st, nullptr); },
                              s);
       if (t >= 0) pick = t;
       // Candidates ran here: start the accumulators again.
@@ -1237,7 +1248,7 @@ void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t
         zero_async(a.z_cp, sizeof(double), s);
       }
     }
-    run(subs_cand[(size_t) pick], false, s, gate);
+    run(subs_cand[pick], false, s, gate);
   } else {
     // Grid-stride over (frame, 256-pixel tile); a chunk is a range of tiles.
     const int64_t tiles = ((a.No + kThreads - 1) / kThreads) * a.n_frames;
@@ -1647,7 +1658,8 @@ void DasHandle::adjoint_dev(const void *Y, const void *Ro, const void *Gt, const
                             double dt, double delay, double cp, const void *apod_t,
                             const void *apod_r, int interp, int64_t n_frames, int64_t y_stride,
                             int64_t im_stride, const void *g, void *gY, void *g_apod_t,
-                            void *g_apod_r, void *g_cp, hipStream_t s)
+                            void *g_apod_r, void *g_cp, hipStream_t s, double *fixed_z,
+                            size_t fixed_z_bytes)
 {
   if (cancelled()) throw Error(DREAM_HIP_ECANCELLED, "das adjoint cancelled");
   require(prec_ != DREAM_HIP_F32_REF, "das adjoint: not available for DREAM_HIP_F32_REF handles");
@@ -1667,17 +1679,26 @@ void DasHandle::adjoint_dev(const void *Y, const void *Ro, const void *Gt, const
           "das adjoint: too many elements for the apodization accumulators");
   // Accumulators: Z per A-scan (region kernel) or z_t/z_r (pixel kernel), z_cp.
   const size_t zbytes = (size_t) (n_ascans + Lt_ + Lr_ + 1) * sizeof(double);
-  if (zbytes > d_z_bytes_) {
-    (void) hipFree(d_z_);
-    d_z_ = nullptr;
-    d_z_bytes_ = 0;
-    if (hipMalloc(&d_z_, zbytes) != hipSuccess)
-      throw Error(DREAM_HIP_ENOMEM, "das adjoint: device allocation failed");
-    d_z_bytes_ = zbytes;
+  double *Z;
+  if (fixed_z) {
+    // Allocation-free call: the caller's fixed workspace, default launch
+    // shapes, one launch per kernel (no chunk events).
+    require(zbytes <= fixed_z_bytes,
+            "das adjoint: n_ascans + Lt + Lr + 1 exceeds the fixed workspace (262144 doubles)");
+    Z = fixed_z;
+  } else {
+    if (zbytes > d_z_bytes_) {
+      (void) hipFree(d_z_);
+      d_z_ = nullptr;
+      d_z_bytes_ = 0;
+      if (hipMalloc(&d_z_, zbytes) != hipSuccess)
+        throw Error(DREAM_HIP_ENOMEM, "das adjoint: device allocation failed");
+      d_z_bytes_ = zbytes;
+    }
+    Z = (double *) d_z_;
   }
-  double *Z = (double *) d_z_;
   double *zt = Z + n_ascans, *zr = zt + Lt_, *zcp = zr + Lr_;
-  zero_async(d_z_, zbytes, s);
+  zero_async(Z, zbytes, s);
   DasArgs a;
   fill_args(a, Y, Ro, Gt, Gr, dt, 0, nullptr, cp, apod_t, apod_r, n_frames, y_stride, im_stride,
             nullptr);
@@ -1689,7 +1710,14 @@ void DasHandle::adjoint_dev(const void *Y, const void *Ro, const void *Gt, const
   a.z_t = channel ? (void *) Z : (void *) zt;
   a.z_r = zr;
   ChunkGate gate(s);
-  if (No_ > 0) launch_das_adjoint(a, type_, prec_, interp, apod, gY != nullptr, s, &gate);
+  if (No_ > 0) {
+    if (fixed_z) {
+      SuspendAutotune no_tuning;
+      launch_das_adjoint(a, type_, prec_, interp, apod, gY != nullptr, s, nullptr);
+    } else {
+      launch_das_adjoint(a, type_, prec_, interp, apod, gY != nullptr, s, &gate);
+    }
+  }
   else if (gY) zero_async(gY, (size_t) (n_frames * y_stride) * elem_size(prec_), s);
   const int n = (int) std::max<int64_t>(std::max<int64_t>(Lt_, Lr_), 1);
   const int blocks = (n + 255) / 256;
@@ -1713,6 +1741,26 @@ void DasHandle::adjoint_dev(const void *Y, const void *Ro, const void *Gt, const
                          (float *) g_cp);
   }
   DREAM_HIP_CHECK(hipGetLastError());
+}
+
+double *fixed_workspace(size_t *bytes)
+{
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  int dev = 0;
+  DREAM_HIP_CHECK(hipGetDevice(&dev));
+  require(dev >= 0 && dev < kMaxDevices, "fixed workspace: device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      throw Error(DREAM_HIP_ERUNTIME, "fixed workspace: hipGetSymbolAddress failed");
+    addr[dev] = p;
+  }
+  *bytes = sizeof(scratch);
+  return static_cast<double *>(addr[dev]);
 }
 
 // ------------------------------------------------------------------ das_uni

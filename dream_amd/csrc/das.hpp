@@ -72,7 +72,8 @@ public:
   void adjoint_dev(const void *Y, const void *Ro, const void *Gt, const void *Gr, double dt,
                    double delay, double cp, const void *apod_t, const void *apod_r, int interp,
                    int64_t n_frames, int64_t y_stride, int64_t im_stride, const void *g,
-                   void *gY, void *g_apod_t, void *g_apod_r, void *g_cp, hipStream_t s);
+                   void *gY, void *g_apod_t, void *g_apod_r, void *g_cp, hipStream_t s,
+                   double *fixed_z = nullptr, size_t fixed_z_bytes = 0);
 
   int type() const { return type_; }
   int precision() const { return prec_; }
@@ -164,6 +165,10 @@ private:
   unsigned long long *d_counter_ = nullptr;
   hipGraphExec_t graph_ = nullptr;
 };
+
+// The allocation-free adjoint's workspace: this device's copy of a fixed
+// device array (its address is looked up once per device) and its size.
+double *fixed_workspace(size_t *bytes);
 
 // With a gate, a long launch runs as chunks with cancellation tests between
 // them (ChunkGate); without one it is a single launch.

@@ -85,11 +85,19 @@ void autotune_clear()
   g_cache.clear();
 }
 
+namespace {
+thread_local bool t_suspended = false;
+} // namespace
+
+SuspendAutotune::SuspendAutotune() : prev_(t_suspended) { t_suspended = true; }
+SuspendAutotune::~SuspendAutotune() { t_suspended = prev_; }
+bool autotune_suspended() { return t_suspended; }
+
 int autotune(const std::string &key, int n, const std::function<void(int, hipStream_t)> &run,
              hipStream_t s)
 {
   if (n <= 1) return n == 1 ? 0 : -1;
-  if (!autotune_enabled()) return -1;
+  if (!autotune_enabled() || t_suspended) return -1;
   const std::string full = arch_name() + "|" + key;
   {
     std::lock_guard<std::mutex> lock(g_mu);

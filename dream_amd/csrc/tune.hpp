@@ -28,6 +28,21 @@ void set_autotune(bool on);
 int autotune(const std::string &key, int n, const std::function<void(int, hipStream_t)> &run,
              hipStream_t s);
 
+// While one of these lives on a thread, launches on that thread use their
+// default shapes: autotune() is not consulted (callers test
+// autotune_suspended() before building its arguments), so nothing is timed,
+// no events are created and no host memory is taken.  For entry points that
+// must not allocate.
+struct SuspendAutotune {
+  SuspendAutotune();
+  ~SuspendAutotune();
+  SuspendAutotune(const SuspendAutotune &) = delete;
+  SuspendAutotune &operator=(const SuspendAutotune &) = delete;
+private:
+  bool prev_;
+};
+bool autotune_suspended();
+
 // Number of cached results (for tests) and removal of all of them.
 int autotune_cache_size();
 void autotune_clear();

@@ -366,6 +366,24 @@ int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const
                               int64_t im_frame_stride, const void *g, void *gY,
                               void *g_apod_t, void *g_apod_r, void *g_cp, void *stream);
 
+/* dream_hip_das_adjoint_dev that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops).  Same arguments and results; the
+ * handle must exist (creating it allocates).  Its accumulators (per-A-scan,
+ * per-element and cp sums, FP64) live in a fixed 2 MiB device array per GPU
+ * instead of handle memory, so n_ascans + Lt + Lr + 1 <= 262144 (larger
+ * problems return DREAM_HIP_EINVAL); default launch shapes (no autotuning)
+ * and one launch per kernel (cancellation is tested on entry only).  The
+ * first call on a device looks the array up (hipGetSymbolAddress).  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU (other
+ * streams, other threads) race. */
+int dream_hip_das_adjoint_dev_noalloc(void *handle, const void *Y, const void *Ro,
+                                      const void *Gt, const void *Gr, double dt, double delay,
+                                      double cp, const void *apod_t, const void *apod_r,
+                                      int interp, int64_t n_frames, int64_t y_frame_stride,
+                                      int64_t im_frame_stride, const void *g, void *gY,
+                                      void *g_apod_t, void *g_apod_r, void *g_cp, void *stream);
+
 /* ---------------------------------------------------------------- das_uni */
 
 /* DAS over implicit uniform grids: DAS_UNI<T>::DAS_UNI / cl_das_uni
