@@ -6,18 +6,27 @@ Headline step: one TFM beamforming of the C2 workload -- 64 x 64 A-scans of
 2048 samples (FP32 data, FP64 delays and sample indices), 256 x 256 pixels --
 with the data already resident in HBM.  value = pixel-channel sums per second
 (N_pix * N_ch / step time).  The JSON line carries:
-  roofline      the DAS forward kernel's algorithmic bytes (in-bounds pairs x 4
-                B + pixels x 4 B) per launch over its HIP-event-timed duration,
-                against 8 TB/s HBM;
+  roofline      the beamforming kernel's algorithmic bytes per launch (the
+                in-bounds pairs it gathers x 4 B + pixels x 4 B; with one
+                array on both sides it gathers each pair (i, j), (j, i) once
+                from A-scans a preparation kernel symmetrizes) over its
+                duration from HIP events recorded around each launch on its
+                stream, against 8 TB/s HBM; the preparation kernel's time and
+                bytes beside it;
   cpu_baseline  the reference's own DAS<float> (das.cc compiled from source,
                 oracle/_ref) or, where that is absent, the oracle's
-                restatement, on the same image on the host cores;
+                restatement, on the same image on the host cores (all
+                threads, and one thread on an eighth of the image), with the
+                CPU model;
   extra         the other measured paths (FP64 / reference-float TFM, the C2
                 image through the host-buffer entry point with its PCIe
                 copies, a 256 MiB device copy, C1 SIR, C3 batched SIR, C4
-                SAFT forward/backward, C5 RCA frames) and, beside the
-                headline's cpu_baseline, the oracle's SIR timed on the host
-                cores (all of C1, a bounded sample of C3).
+                SAFT forward/backward, C5 RCA frames, the C2 image summing
+                every pair on its own, and with the out-of-bounds status read
+                back each call) and, beside the headline's cpu_baseline, the
+                reference's DAS<float> on C4 and a C5 sub-volume and the
+                oracle's SIR on the host cores (all of C1, a bounded sample of
+                C3).
 Only one GPU is used; --gpus N > 1 exits with a message.
 """
 import argparse
@@ -46,6 +55,42 @@ def measured_traffic():
         pass
     return None
 METRIC = "DAS pixel·channel sums/s + SIR obs·elem/s vs FP64 CPU; %HBM roofline; bwd/fwd"
+# MI355X: 256 CUs at 2.4 GHz; a CU's 4 SIMDs retire 64 VALU lane-operations
+# per cycle (a wave instruction takes 4 cycles); one L1 tag lookup per CU per
+# cycle.
+N_CU, CLOCK_HZ = 256, 2.4e9
+C3_COUNTERS = os.path.join(ROOT, "profiles", "sir_kernel_c3_counters.md")
+C5_COUNTERS = os.path.join(ROOT, "profiles", "das_rca_c5_counters.md")
+
+
+def _counter_value(path, key):
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def valu_issue_frac(contributions, ms):
+    """C3: VALU lane-instructions per second (the committed counters' lane
+    instructions per contribution x contributions / time) over the device's
+    rate (256 CUs x 64 lanes x 2.4 GHz), or None."""
+    per = _counter_value(C3_COUNTERS, "valu_lane_insts_per_contribution")
+    if per is None:
+        return None
+    return per * contributions / (ms * 1e-3) / (N_CU * 64 * CLOCK_HZ)
+
+
+def l1_lookup_frac(pairs, ms):
+    """C5: L1 tag lookups per second (the committed counters' lookups per
+    64-lane gather x gathers / time) over one lookup per CU per cycle, or None."""
+    per = _counter_value(C5_COUNTERS, "tag_lookups_per_gather")
+    if per is None:
+        return None
+    return per * (pairs / 64) / (ms * 1e-3) / (N_CU * CLOCK_HZ)
 
 
 def log(msg):
@@ -74,6 +119,28 @@ def cpu_threads():
         return os.cpu_count() or 1
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def _median_time(fn, runs=3):
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return sorted(ts)[len(ts) // 2]
+
+
 def bench_c2(steps, warmup, want_cpu, want_extra=True):
     import numpy as np
     import torch
@@ -85,20 +152,58 @@ def bench_c2(steps, warmup, want_cpu, want_extra=True):
     Y = w["Y"].contiguous()
     plan = DasPlan("tfm", w["K"], w["Ro"], w["Gt"], w["Gr"], precision="f32")
     n_pix, n_ch = plan.No, plan.n_ascans
+    L = plan.Lt
     out = torch.empty(n_pix, dtype=torch.float32, device="cuda")
-    # Exact in-bounds pair count: beamform all-ones data (integer sums, exact in FP32).
+    # Exact in-bounds pair counts: beamform all-ones data (integer sums, exact
+    # in FP32), and data that is one on the A-scans with transmit == receive.
     ones = torch.ones_like(Y)
     n_inb = int(plan.forward(ones, w["dt"], 0.0, w["cp"]).double().sum().item())
+    ones.zero_()
+    ones[torch.arange(L, device="cuda") * (L + 1)] = 1.0
+    n_diag = int(plan.forward(ones, w["dt"], 0.0, w["cp"]).double().sum().item())
     del ones
+    # One array on both sides and a zero delay: the forward gathers each
+    # pair (i, j), (j, i) once from the symmetrized A-scans.
+    reciprocal = plan.Gr is plan.Gt
+    gathered = (n_inb - n_diag) // 2 + n_diag if reciprocal else n_inb
     step = lambda: plan.forward(Y, w["dt"], 0.0, w["cp"], out=out)  # noqa: E731
     log(f"C2 TFM f32: warmup {warmup}, timing {steps} steps")
     ms = time_launches(step, steps, warmup)
     sums_per_s = n_pix * n_ch / (ms * 1e-3)
-    alg_bytes = n_inb * 4 + n_pix * 4
-    achieved = alg_bytes / (ms * 1e-3)
-    res = dict(ms=ms, value=sums_per_s, n_inb=n_inb, alg_bytes=alg_bytes, achieved=achieved)
+    # The beamforming kernel's own duration: HIP events recorded on its
+    # stream around each launch, over a second pass of the same steps.
+    plan.time_kernels(steps)
+    for _ in range(steps):
+        step()
+    n_t, prep_total, fwd_total = plan.kernel_times()
+    kernel_ms = fwd_total / n_t
+    prep_ms = prep_total / n_t
+    alg_bytes = gathered * 4 + n_pix * 4
+    prep_bytes = (n_ch * w["K"] + (L * (L + 1) // 2) * w["K"]) * 4 if reciprocal else 0
+    achieved = alg_bytes / (kernel_ms * 1e-3)
+    res = dict(ms=ms, value=sums_per_s, n_inb=n_inb, n_diag=n_diag, gathered=gathered,
+               alg_bytes=alg_bytes, achieved=achieved, kernel_ms=kernel_ms, prep_ms=prep_ms,
+               prep_bytes=prep_bytes, reciprocal=reciprocal)
 
     extra = {}
+    if want_extra:
+        # The same image summing every (transmit, receive) pair on its own:
+        # the receive array passed as a separate copy, so the reciprocal
+        # pairing (one gather per pair (i, j), (j, i)) is not used.
+        full = DasPlan("tfm", w["K"], w["Ro"], w["Gt"], w["Gr"], precision="f32")
+        full.Gr = full.Gt.clone()
+        m = time_launches(lambda: full.forward(Y, w["dt"], 0.0, w["cp"], out=out), steps, warmup)
+        extra["c2_tfm_f32_full_pair_sum"] = dict(ms=round(m, 4),
+                                                 sums_per_s=n_pix * n_ch / (m * 1e-3))
+        del full
+        # The reference's das() returns the out-of-bounds status
+        # (extra_src/das.cc:338-351): the kernel collects it and each call
+        # reads it back (one synchronisation per call).
+        m = time_launches(lambda: plan.forward(Y, w["dt"], 0.0, w["cp"], out=out,
+                                               err_level="ignore", want_status=True),
+                          steps, warmup)
+        extra["c2_tfm_f32_with_status"] = dict(ms=round(m, 4),
+                                               sums_per_s=n_pix * n_ch / (m * 1e-3))
     # FP64 and the reference-float path on the same image.
     for prec, dtype in ((("f64", torch.float64), ("f32_ref", torch.float32)) if want_extra else ()):
         p = DasPlan("tfm", w["K"], w["Ro"], w["Gt"], w["Gr"], precision=prec)
@@ -210,10 +315,13 @@ def bench_sir(steps, warmup):
                      p["v"], p["cp"], p["G"], foc_met="ud", focal=p["focal"], apod_met="hann",
                      err_level="ignore", out_dtype=torch.float32, out=H)
     ms_p = time_launches(per_call, max(1, steps // 25), 1)
+    # The SIR kernel is bound by FP64 VALU issue, not HBM (its histograms
+    # live in LDS): the rate is reported against that bound from the
+    # committed counter summary, not as an HBM fraction.
     out["c3_sir_batched_f32"] = dict(
         ms=round(ms_b, 3), obs_elem_per_s=n_oe / (ms_b * 1e-3),
         contributions_per_s=contrib / (ms_b * 1e-3), in_bounds_contributions=int(inb),
-        roofline_frac=(inb * 8 + out_bytes) / (ms_b * 1e-3) / HBM_PEAK,
+        bound="VALU issue", valu_issue_frac=valu_issue_frac(contrib, ms_b),
         per_call_ms=round(ms_p, 3), speedup_vs_per_call=ms_p / ms_b)
     out.update(bench_annu(steps, warmup))
     out.update(bench_att(steps, warmup))
@@ -407,32 +515,90 @@ def bench_das_diff_c5(steps, warmup):
     pairs = plan.No * plan.n_ascans * F
     n_inb = int(plan.forward(torch.ones_like(Y), w["dt"], 0.0, w["cp"]).double().sum().item())
     alg_bytes = n_inb * 4 + F * plan.No * 4
+    # Y (268 MB per frame) is re-read from L2/MALL by every voxel tile, so the
+    # no-reuse byte count exceeds HBM bandwidth: it is not a bound for C5.
+    # The forward is bound by the L1 tag-lookup rate (one per CU cycle).
     out["c5_rca_8frames_f32"] = dict(fwd_ms=round(fwd, 3), bwd_apod_ms=round(bwd, 3),
                                      bwd_over_fwd=bwd / fwd, sums_per_s=pairs / (fwd * 1e-3),
-                                     in_bounds_pairs=n_inb,
-                                     roofline_frac=alg_bytes / (fwd * 1e-3) / HBM_PEAK)
+                                     in_bounds_pairs=n_inb, bound="L1 tag lookups",
+                                     no_reuse_bytes_per_s=alg_bytes / (fwd * 1e-3),
+                                     l1_lookup_frac=l1_lookup_frac(pairs, fwd))
     return out
 
 
-def cpu_baseline_c2(w, n_pix, n_ch):
-    import numpy as np
+def _cpu_das(Y, Gt, Gr, Ro, dt, cp, method, threads, use_ref):
+    """One DAS<float> call on the host: the reference's das.cc compiled from
+    source (use_ref) or the oracle's restatement, `threads` threads."""
+    from oracle import oracle as O
+    os.environ["DREAM_NUM_THREADS"] = str(threads)
+    O.das(Y, Gt, Gr, Ro, dt, 0.0, cp, method, "ignore", "f32", nthreads=threads,
+          use_ref=use_ref)
 
+
+def cpu_baseline_c2(w, n_pix, n_ch):
+    """The reference's DAS<float> (das.cc compiled from source; the oracle's
+    restatement where that checker is absent) on the whole C2 image with all
+    host threads, median of 3; and on one thread over the first 8192 pixels
+    (1/8 of the image), median of 3, in the same unit."""
     from oracle import oracle as O
     threads = cpu_threads()
-    os.environ["DREAM_NUM_THREADS"] = str(threads)
     use_ref = O.have_ref()
     Y = w["Y"].cpu().numpy().T.copy()
-    times = []
-    for _ in range(3):
-        t = time.perf_counter()
-        O.das(Y, w["Gt"], w["Gr"], w["Ro"], w["dt"], 0.0, w["cp"], "tfm", "ignore", "f32",
-              nthreads=threads, use_ref=use_ref)
-        times.append(time.perf_counter() - t)
-    t = sorted(times)[1]
+    t = _median_time(lambda: _cpu_das(Y, w["Gt"], w["Gr"], w["Ro"], w["dt"], w["cp"], "tfm",
+                                      threads, use_ref))
+    n1 = 8192
+    t1 = _median_time(lambda: _cpu_das(Y, w["Gt"], w["Gr"], w["Ro"][:n1], w["dt"], w["cp"],
+                                       "tfm", 1, use_ref))
     return {"value": n_pix * n_ch / t, "unit": "pixel·channel sums/s", "cores": threads,
             "kind": "reference" if use_ref else "port",
             "sample": f"full C2 TFM image ({n_pix} pixels x {n_ch} A-scans), DAS<float> on the "
-                      f"host, {threads} threads, median of 3"}
+                      f"host, {threads} threads, median of 3",
+            "cpu_model": cpu_model(),
+            "value_1_thread": n1 * n_ch / t1,
+            "sample_1_thread": f"first {n1} pixels of the C2 image x {n_ch} A-scans, 1 thread, "
+                               "median of 3"}
+
+
+def cpu_baselines_c4_c5():
+    """C4 and C5 on the host cores, the reference's DAS<float> in its own
+    (reference) mode: nearest-lower sample, no apodization or interpolation,
+    forward only.  C4: the whole 512 x 512 SAFT image.  C5: one frame over a
+    16 x 16 x 256-voxel sub-volume of the 128 x 128 x 256 grid (x, y indices
+    56..71), extrapolated linearly in voxels and frames to the 8-frame volume
+    (x 512), labelled as such."""
+    import numpy as np
+    import torch
+
+    import workloads
+    from oracle import oracle as O
+    threads = cpu_threads()
+    use_ref = O.have_ref()
+    kind = "reference" if use_ref else "port"
+    out = {}
+    w = workloads.c4(device="cpu", dtype=torch.float32)
+    Y = w["Y"].numpy().T.copy()
+    n_pc = w["Ro"].shape[0] * w["Gt"].shape[0]
+    t = _median_time(lambda: _cpu_das(Y, w["Gt"], None, w["Ro"], w["dt"], w["cp"], "saft",
+                                      threads, use_ref))
+    out["c4_saft"] = dict(ms=round(t * 1e3, 3), sums_per_s=n_pc / t, cores=threads, kind=kind,
+                          sample=f"full C4 image ({w['Ro'].shape[0]} pixels x 128 A-scans), "
+                                 f"nearest-lower, {threads} threads, median of 3")
+    w = workloads.c5(device="cpu", dtype=torch.float32, n_frames=1)
+    nx, ny, nz = 128, 128, 256
+    ix, iy, iz = np.meshgrid(np.arange(56, 72), np.arange(56, 72), np.arange(nz), indexing="ij")
+    sub = w["Ro"][(ix + nx * (iy + ny * iz)).ravel("F")]
+    Y = w["Y"][0].numpy().T.copy()
+    n_ch = w["Gt"].shape[0] * w["Gr"].shape[0]
+    t = _median_time(lambda: _cpu_das(Y, w["Gt"], w["Gr"], sub, w["dt"], w["cp"], "rca_rowtx",
+                                      threads, use_ref), runs=1)
+    scale = (nx * ny * nz * 8) / sub.shape[0]
+    out["c5_rca_rowtx"] = dict(ms=round(t * 1e3, 3), sums_per_s=sub.shape[0] * n_ch / t,
+                               cores=threads, kind=kind,
+                               extrapolated_8_frames_s=round(t * scale, 2),
+                               sample=f"one frame, {sub.shape[0]} voxels (16 x 16 x 256 of the "
+                                      f"grid) x {n_ch} A-scans, {threads} threads, one run; "
+                                      f"8 frames x full grid extrapolated x{scale:.0f}")
+    return out
 
 
 def cpu_baseline_c1():
@@ -534,6 +700,7 @@ def main():
         if not args.no_cpu:
             extra["c1_sir_cpu_baseline"] = cpu_baseline_c1()
             extra["c3_sir_cpu_sample"] = cpu_baseline_c3_sample()
+            extra["cpu_baselines"] = cpu_baselines_c4_c5()
     line = {
         "metric": METRIC,
         "value": res["value"],
@@ -550,10 +717,16 @@ def main():
         "config": {"workload": "C2 TFM das: 64 x 64 A-scans x 2048 samples at 50 MHz, 256 x 256 "
                                "pixels at 0.125 mm, FP32 data with FP64 delays, nearest-lower "
                                "sample (reference rule)",
-                   "n_pixels": 65536, "n_channels": 4096, "in_bounds_pairs": res["n_inb"]},
+                   "n_pixels": 65536, "n_channels": 4096, "in_bounds_pairs": res["n_inb"],
+                   "gathered_in_bounds_pairs": res["gathered"],
+                   "reciprocal_pairs": res["reciprocal"]},
         "roofline": {"bound": "hbm", "achieved": res["achieved"] / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": res["achieved"] / HBM_PEAK,
-                     "traffic": measured_traffic()},
+                     "traffic": measured_traffic(),
+                     "kernel": "das_fwd_kernel<float, double, TFM, nearest>",
+                     "kernel_ms": res["kernel_ms"],
+                     "prep_kernel": "das_symmetrize_kernel<float, double>",
+                     "prep_kernel_ms": res["prep_ms"], "prep_bytes": res["prep_bytes"]},
         "cpu_baseline": cpu,
     }
     if not args.no_extra:

@@ -37,7 +37,8 @@ EXPORTS = (
     "dream_hip_das_create", "dream_hip_das_destroy", "dream_hip_das_setup_has_changed",
     "dream_hip_das_set_pixel_order", "dream_hip_pixel_order",
     "dream_hip_das", "dream_hip_das_dev", "dream_hip_das_adjoint_dev",
-    "dream_hip_das_adjoint_dev_noalloc",
+    "dream_hip_das_adjoint_dev_noalloc", "dream_hip_das_time_kernels",
+    "dream_hip_das_kernel_times",
     "dream_hip_das_batch_create", "dream_hip_das_batch_run", "dream_hip_das_batch_destroy",
     "dream_hip_fftconv_p", "dream_hip_sum_fftconv", "dream_hip_conv_sum_dev",
     "dream_hip_das_uni_create", "dream_hip_das_uni_destroy", "dream_hip_das_uni_shape",
@@ -201,6 +202,10 @@ def _declare(L):
     L.dream_hip_das_adjoint_dev.restype = _i32
     L.dream_hip_das_adjoint_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _vp, _vp,
                                             _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.dream_hip_das_time_kernels.restype = _i32
+    L.dream_hip_das_time_kernels.argtypes = [_vp, _i32]
+    L.dream_hip_das_kernel_times.restype = _i32
+    L.dream_hip_das_kernel_times.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _dp, _dp]
     L.dream_hip_das_adjoint_dev_noalloc.restype = _i32
     L.dream_hip_das_adjoint_dev_noalloc.argtypes = L.dream_hip_das_adjoint_dev.argtypes
 

@@ -648,6 +648,24 @@ int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const
   });
 }
 
+int dream_hip_das_time_kernels(void *handle, int n_max)
+{
+  return guarded([&]() {
+    require(handle != nullptr, "das timing: null handle");
+    static_cast<DasHandle *>(handle)->start_timing(n_max);
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_kernel_times(void *handle, int *n, double *prep_ms, double *forward_ms)
+{
+  return guarded([&]() {
+    require(handle != nullptr, "das timing: null handle");
+    static_cast<DasHandle *>(handle)->read_timing(n, prep_ms, forward_ms);
+    return DREAM_HIP_OK;
+  });
+}
+
 int dream_hip_das_adjoint_dev_noalloc(void *handle, const void *Y, const void *Ro,
                                       const void *Gt, const void *Gr, double dt, double delay,
                                       double cp, const void *apod_t, const void *apod_r,

@@ -29,6 +29,7 @@
 // weights and cp.
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <cmath>
 #include <cstdio>
@@ -223,7 +224,8 @@ constexpr int kMaxWaves = 8;     // waves per forward work-group (64 .. 512 thre
 // pixel at visiting position idx (one lane per pixel, 64 per wave; every
 // lane of the wave runs the same jobs).  `oob` is set when a pair of the
 // pixel fell outside [0, K) (only tracked when STATUS).
-template <typename DT, typename GT, int TYPE, int INTERP, bool APOD, bool STATUS>
+template <typename DT, typename GT, int TYPE, int INTERP, bool APOD, bool STATUS,
+          bool RECIP = false>
 __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__restrict__ perm,
                                            int64_t idx, int64_t frame, int job0, int jstride,
                                            bool &oob)
@@ -238,7 +240,13 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
   const GT tmax = (GT) A.leg_max;
   const int K = (int) A.K;
   const unsigned Ku = (unsigned) K;
-  const DT *Yf = (const DT *) A.Y + frame * A.y_stride;
+  // Reciprocal TFM (see DasArgs::Ysym; the host sets it only for a scalar
+  // delay): the test reads the delay at one address and is made explicitly
+  // wave-uniform, so the job loop below stays scalar.
+  bool recip = false;
+  if (RECIP && TYPE == DAS_TFM)
+    recip = __builtin_amdgcn_readfirstlane((int) (delay_ms_of<GT>(gdel[0]) == GT(0))) != 0;
+  const DT *Yf = (const DT *) (recip ? A.Ysym : A.Y) + frame * A.y_stride;
   const __amdgpu_buffer_rsrc_t rY = frame_rsrc(Yf, A.y_stride * (int64_t) sizeof(DT));
   const GT *Gt = (const GT *) A.Gt;
   const GT *Gr = (const GT *) A.Gr;
@@ -301,9 +309,20 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
     }
     const bool delay_rx = A.delay_rx != 0;
     const int nchunks = (Lr + kRC - 1) / kRC;
-    const int J = nchunks * ((Lt + kTB - 1) / kTB);
+    // Reciprocal TFM visits the blocks b <= c only (Lt == Lr, kTB == kRC).
+    const int J = recip ? nchunks * (nchunks + 1) / 2 : nchunks * ((Lt + kTB - 1) / kTB);
     for (int job = job0; job < J; job += jstride) {
-      const int c = job % nchunks, b = job / nchunks;
+      int c = job % nchunks, b = job / nchunks;
+      if (recip) {
+        b = 0;
+        c = job;
+        while (c >= nchunks - b) {
+          c -= nchunks - b;
+          b++;
+        }
+        c += b;
+      }
+      const bool diag = recip && b == c;     // in a diagonal block only j >= i
       const int j0 = c * kRC, i0 = b * kTB, i1 = min(i0 + kTB, Lt);
       const bool full = j0 + kRC <= Lr;
       GT tr[kRC];
@@ -353,7 +372,7 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
 #pragma unroll
           for (int jj = 0; jj < kG; jj++) {
             const int j = min(j0 + g0 + jj, Lr - 1);
-            const bool live = full || j0 + g0 + jj < Lr;
+            const bool live = (full || j0 + g0 + jj < Lr) && (!diag || j0 + g0 + jj >= i);
             const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
             int k;
             gather_offsets<DT, GT, INTERP>((unsigned) a * Ku, Ku, (tt + tr[g0 + jj]) * fs, live,
@@ -378,7 +397,8 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
   return acc;
 }
 
-template <typename DT, typename GT, int TYPE, int INTERP, bool APOD, bool STATUS>
+template <typename DT, typename GT, int TYPE, int INTERP, bool APOD, bool STATUS,
+          bool RECIP = false>
 __global__ void __launch_bounds__(64 * kMaxWaves)
 das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__restrict__ partial)
 {
@@ -394,7 +414,7 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
   const bool valid = idx < A.No;
   const int64_t p = valid ? pixel_of(perm, idx) : 0;
   bool oob = false;
-  const DT acc = fwd_tile_sum<DT, GT, TYPE, INTERP, APOD, STATUS>(
+  const DT acc = fwd_tile_sum<DT, GT, TYPE, INTERP, APOD, STATUS, RECIP>(
       A, perm, idx, frame, sidx * nwaves + wave, nwaves * split, oob);
   // Sum the waves' partial sums in a fixed order.
   red[wave][lane] = acc;
@@ -411,17 +431,63 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
   }
 }
 
-// Im[f][p] = sum over s of partial[s][f][p], s ascending (split launches).
+// Im[f][p] = sum over s of partial[s][f][p], s ascending (split launches),
+// for the pixels at visiting-order indices [idx0, idx0 + n_idx) of frames
+// [frame0, frame0 + nf): the pixels of one chunk of the forward launch.
 template <typename DT>
 __global__ void das_reduce_kernel(const DT *__restrict__ partial, int split, int64_t No,
-                                  int64_t n_frames, int64_t im_stride, DT *__restrict__ Im)
+                                  int64_t n_frames, int64_t frame0, int64_t nf, int64_t idx0,
+                                  int64_t n_idx, const int32_t *__restrict__ perm,
+                                  int64_t im_stride, DT *__restrict__ Im)
 {
   const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= No * n_frames) return;
-  const int64_t f = q / No, p = q - f * No;
-  DT sum = partial[q];
-  for (int s = 1; s < split; s++) sum += partial[(int64_t) s * n_frames * No + q];
+  if (q >= n_idx * nf) return;
+  const int64_t f = frame0 + q / n_idx, idx = idx0 + q % n_idx;
+  if (idx >= No) return;
+  const int64_t p = pixel_of(perm, idx);
+  const int64_t at = f * No + p;
+  DT sum = partial[at];
+  for (int s = 1; s < split; s++) sum += partial[(int64_t) s * n_frames * No + at];
   Im[f * im_stride + p] = sum;
+}
+
+// Ysym[f][i L + j] = Y[f][i L + j] + Y[f][j L + i] (i < j), Y[f][i L + i]
+// (i == j), for the reciprocal TFM forward; nothing when the scalar delay
+// read on the device is not zero (the forward then reads Y).  One
+// work-group per (j, i, frame); i > j exit at once.  16-byte accesses when
+// K and the frame stride keep every A-scan 16-byte aligned.
+template <typename DT, typename GT>
+__global__ void __launch_bounds__(256)
+das_symmetrize_kernel(const DT *__restrict__ Y, DT *__restrict__ Ys, int L, int64_t K,
+                      int64_t y_stride, const GT *__restrict__ delay)
+{
+  const int j = blockIdx.x, i = blockIdx.y;
+  if (i > j || delay_ms_of<GT>(delay[0]) != GT(0)) return;
+  const int64_t f = blockIdx.z;
+  const DT *a = Y + f * y_stride + ((int64_t) i * L + j) * K;
+  const DT *b = Y + f * y_stride + ((int64_t) j * L + i) * K;
+  DT *o = Ys + f * y_stride + ((int64_t) i * L + j) * K;
+  constexpr int V = 16 / sizeof(DT);
+  const bool vec = (K % V) == 0 && (y_stride % V) == 0 &&
+                   ((reinterpret_cast<uintptr_t>(Y) | reinterpret_cast<uintptr_t>(Ys)) & 15) == 0;
+  if (vec) {
+    using V4 = typename std::conditional<sizeof(DT) == 4, float4, double2>::type;
+    const V4 *av = reinterpret_cast<const V4 *>(a);
+    const V4 *bv = reinterpret_cast<const V4 *>(b);
+    V4 *ov = reinterpret_cast<V4 *>(o);
+    const int64_t n = K / V;
+    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
+      V4 x = av[k];
+      if (i != j) x = x + bv[k];
+      ov[k] = x;
+    }
+    return;
+  }
+  if (i == j) {
+    for (int64_t k = threadIdx.x; k < K; k += blockDim.x) o[k] = a[k];
+  } else {
+    for (int64_t k = threadIdx.x; k < K; k += blockDim.x) o[k] = a[k] + b[k];
+  }
 }
 
 // ---------------------------------------------------------------- uneven batch
@@ -1006,6 +1072,17 @@ bool rca_extents_in_bounds(int64_t Lt, int64_t Lr)
   return 2 * Lt <= 3 * Lr && Lr <= 3 * Lt;
 }
 
+// DREAM_HIP_RECIPROCAL=0 switches the reciprocal TFM forward off (the full
+// pair sum runs), for comparisons.
+bool recip_enabled()
+{
+  static const bool on = [] {
+    const char *e = std::getenv("DREAM_HIP_RECIPROCAL");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 int cu_count()
 {
   static int cus = 0;
@@ -1027,12 +1104,20 @@ struct FwdShape {
   int waves, split;
 };
 
-FwdShape forward_shape(int type, int64_t No, int64_t n_frames, int64_t Lt, int64_t Lr)
+// Jobs per 64-pixel tile.
+int64_t forward_jobs(int type, int64_t Lt, int64_t Lr, bool recip)
+{
+  if (type == DAS_SAFT) return (Lt + kSB - 1) / kSB;
+  const int64_t nc = (Lr + kRC - 1) / kRC;
+  return recip ? nc * (nc + 1) / 2 : nc * ((Lt + kTB - 1) / kTB);
+}
+
+FwdShape forward_shape(int type, int64_t No, int64_t n_frames, int64_t Lt, int64_t Lr,
+                       bool recip = false)
 {
   const int64_t tiles = ((No + 63) / 64) * n_frames;
   const int64_t target = (int64_t) cu_count() * 32;      // 8 waves per SIMD
-  const int64_t J = type == DAS_SAFT ? (Lt + kSB - 1) / kSB
-                                     : ((Lr + kRC - 1) / kRC) * ((Lt + kTB - 1) / kTB);
+  const int64_t J = forward_jobs(type, Lt, Lr, recip);
   FwdShape sh{4, 1};
   if (tiles == 0) return sh;                              // no pixels: nothing launches
   if (tiles * 4 < target) sh.waves = kMaxWaves;
@@ -1048,14 +1133,20 @@ FwdShape forward_shape(int type, int64_t No, int64_t n_frames, int64_t Lt, int64
 
 // Candidate launch shapes for the autotuner; the heuristic shape first.
 std::vector<FwdShape> forward_candidates(int type, int64_t No, int64_t n_frames, int64_t Lt,
-                                         int64_t Lr)
+                                         int64_t Lr, bool recip = false)
 {
-  const FwdShape h = forward_shape(type, No, n_frames, Lt, Lr);
-  const int64_t J = type == DAS_SAFT ? (Lt + kSB - 1) / kSB
-                                     : ((Lr + kRC - 1) / kRC) * ((Lt + kTB - 1) / kTB);
+  const FwdShape h = forward_shape(type, No, n_frames, Lt, Lr, recip);
+  const int64_t J = forward_jobs(type, Lt, Lr, recip);
   std::vector<FwdShape> c{h};
   const int splits[3] = {1, h.split, 2 * h.split};
-  for (int w : {2, 4, 8})
+  // 2, 4, 8 waves, and the wave counts that deal J jobs evenly (ceil(J / k)).
+  std::vector<int> waves{2, 4, 8};
+  for (int64_t k = 1; k <= 4; k++) {
+    const int64_t w = (J + k - 1) / k;
+    if (w >= 2 && w <= kMaxWaves && std::find(waves.begin(), waves.end(), (int) w) == waves.end())
+      waves.push_back((int) w);
+  }
+  for (int w : waves)
     for (int sp : splits) {
       if (w > std::max<int64_t>(J, 1) || sp > 64) continue;
       if ((int64_t) w * sp > std::max<int64_t>(J, 1) * 2) continue;   // most waves idle
@@ -1113,29 +1204,42 @@ void fwd_launch_shape(const DasArgs &a0, FwdShape sh, hipStream_t s, ChunkGate *
     sh.split = 1;
   dim3 block((unsigned) (64 * sh.waves));
   const int64_t tiles = (a0.No + 63) / 64;
+  // A chunk is a range of tiles of some frames; with split partial sums its
+  // pixels are reduced right after it, so every launched chunk's pixels are
+  // written when a later chunk is cancelled.
   auto launch = [&](int64_t b0, int64_t nb, int64_t f0, int64_t nf) {
     DasArgs a = a0;
     a.block0 = b0;
     a.frame0 = f0;
     dim3 grid((unsigned) nb, (unsigned) sh.split, (unsigned) nf);
-    if (a.status)
+    // The reciprocal TFM kernel is its own instance (no apodization), so the
+    // other kernels keep their registers.
+    constexpr bool kRecip = TYPE == DAS_TFM && !APOD;
+    if (kRecip && a.recip) {
+      if (a.status)
+        hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, true, kRecip>), grid,
+                           block, 0, s, a, a.perm, sh.split, (DT *) a.scratch);
+      else
+        hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, false, kRecip>), grid,
+                           block, 0, s, a, a.perm, sh.split, (DT *) a.scratch);
+    } else if (a.status) {
       hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, true>), grid, block, 0, s,
                          a, a.perm, sh.split, (DT *) a.scratch);
-    else
+    } else {
       hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, false>), grid, block, 0, s,
                          a, a.perm, sh.split, (DT *) a.scratch);
+    }
     DREAM_HIP_CHECK(hipGetLastError());
+    if (sh.split > 1) {
+      const int64_t n_idx = std::min<int64_t>(nb * 64, a.No - b0 * 64);
+      const int64_t n = n_idx * nf;
+      hipLaunchKernelGGL(das_reduce_kernel<DT>, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0,
+                         s, (const DT *) a.scratch, sh.split, a.No, a.n_frames, f0, nf, b0 * 64,
+                         n_idx, a.perm, a.im_stride, (DT *) a.Im);
+      DREAM_HIP_CHECK(hipGetLastError());
+    }
   };
-  // Split launches are small images (the grid cannot fill the device): one chunk.
-  run_chunked(tiles, a0.n_frames, 64 * pairs_per_pixel(TYPE, a0.Lt, a0.Lr),
-              sh.split > 1 ? nullptr : gate, launch);
-  const DasArgs &a = a0;
-  if (sh.split > 1) {
-    const int64_t n = a.No * a.n_frames;
-    hipLaunchKernelGGL(das_reduce_kernel<DT>, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0,
-                       s, (const DT *) a.scratch, sh.split, a.No, a.n_frames, a.im_stride,
-                       (DT *) a.Im);
-  }
+  run_chunked(tiles, a0.n_frames, 64 * pairs_per_pixel(TYPE, a0.Lt, a0.Lr), gate, launch);
 }
 
 // The heuristic shape, or the autotuner's pick for this shape class (not
@@ -1143,13 +1247,14 @@ void fwd_launch_shape(const DasArgs &a0, FwdShape sh, hipStream_t s, ChunkGate *
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
 void fwd_launch(const DasArgs &a, hipStream_t s, ChunkGate *gate)
 {
-  const std::vector<FwdShape> cand = forward_candidates(TYPE, a.No, a.n_frames, a.Lt, a.Lr);
+  const std::vector<FwdShape> cand =
+      forward_candidates(TYPE, a.No, a.n_frames, a.Lt, a.Lr, a.recip != 0);
   int pick = 0;
   if (!a.status && cand.size() > 1) {
     char key[192];
-    std::snprintf(key, sizeof key, "das_fwd|%d|%d|%d|%d|%d|%lld|%d|%d|%lld|%lld", TYPE,
+    std::snprintf(key, sizeof key, "das_fwd|%d|%d|%d|%d|%d|%lld|%d|%d|%lld|%lld|%d", TYPE,
                   (int) sizeof(DT), (int) sizeof(GT), INTERP, (int) APOD, (long long) a.No, a.Lt,
-                  a.Lr, (long long) a.K, (long long) a.n_frames);
+                  a.Lr, (long long) a.K, (long long) a.n_frames, (int) a.recip);
     const int t = autotune(key, (int) cand.size(),
                            [&](int i, hipStream_t st) { fwd_launch_shape<DT, GT, TYPE, INTERP, APOD>(a, cand[(size_t) i], st); },
                            s);
@@ -1362,8 +1467,10 @@ size_t das_forward_scratch_bytes(int type, int prec, int64_t No, int64_t n_frame
                                  int64_t Lr)
 {
   int split = 1;
-  for (const FwdShape &sh : forward_candidates(Lr == 0 ? DAS_SAFT : type, No, n_frames, Lt, Lr))
-    split = std::max(split, sh.split);
+  for (bool recip : {false, true})
+    for (const FwdShape &sh : forward_candidates(Lr == 0 ? DAS_SAFT : type, No, n_frames, Lt, Lr,
+                                                 recip))
+      split = std::max(split, sh.split);
   return split > 1 ? (size_t) split * (size_t) n_frames * (size_t) No * elem_size(prec) : 0;
 }
 
@@ -1484,6 +1591,8 @@ DasHandle::DasHandle(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, in
 
 DasHandle::~DasHandle()
 {
+  drop_timing();
+  (void) hipFree(d_sym_);
   (void) hipFree(d_status_);
   (void) hipHostFree(h_status_);
   (void) hipFree(d_buf_);
@@ -1495,6 +1604,39 @@ DasHandle::~DasHandle()
 void DasHandle::ensure_scratch_for(int64_t n_frames)
 {
   ensure_scratch(das_forward_scratch_bytes(type_, prec_, No_, n_frames, Lt_, Lr_));
+}
+
+void DasHandle::drop_timing()
+{
+  for (hipEvent_t e : tev_) (void) hipEventDestroy(e);
+  tev_.clear();
+  t_cap_ = t_n_ = 0;
+}
+
+void DasHandle::start_timing(int n_max)
+{
+  require(n_max >= 1 && n_max <= 100000, "das timing: n_max out of range (1 .. 100000)");
+  drop_timing();
+  tev_.assign((size_t) 3 * n_max, nullptr);
+  for (hipEvent_t &e : tev_) DREAM_HIP_CHECK(hipEventCreate(&e));
+  t_cap_ = n_max;
+}
+
+void DasHandle::read_timing(int *n, double *prep_ms, double *forward_ms)
+{
+  double p = 0.0, f = 0.0;
+  for (int i = 0; i < t_n_; i++) {
+    float a = 0.0f, b = 0.0f;
+    DREAM_HIP_CHECK(hipEventSynchronize(tev_[(size_t) 3 * i + 2]));
+    DREAM_HIP_CHECK(hipEventElapsedTime(&a, tev_[(size_t) 3 * i], tev_[(size_t) 3 * i + 1]));
+    DREAM_HIP_CHECK(hipEventElapsedTime(&b, tev_[(size_t) 3 * i + 1], tev_[(size_t) 3 * i + 2]));
+    p += a;
+    f += b;
+  }
+  if (n) *n = t_n_;
+  if (prep_ms) *prep_ms = p;
+  if (forward_ms) *forward_ms = f;
+  drop_timing();
 }
 
 void DasHandle::ensure_scratch(size_t bytes)
@@ -1587,8 +1729,44 @@ int DasHandle::run_dev(const void *Y, const void *Ro, const void *Gt, const void
             im_stride, Im);
   if (want_status) zero_async(d_status_, sizeof(Status), s);
   else a.status = nullptr;
+  const bool timed = t_n_ < t_cap_ && No_ > 0;
+  if (timed) DREAM_HIP_CHECK(hipEventRecord(tev_[(size_t) 3 * t_n_], s));
+  // Reciprocal TFM: one array for both sides (the same pointer), one scalar
+  // delay, no apodization.  The pairs (i, j) and (j, i) then share their
+  // sample index whenever the delay is zero, which the symmetrize kernel and
+  // the forward test on the device; Ysym holds the pair sums.
+  if (No_ > 0 && type_ == DAS_TFM && Gt == Gr && Lt_ == Lr_ && delay_type == 0 && !apod &&
+      recip_enabled()) {
+    const size_t bytes = (size_t) (n_frames * y_stride) * elem_size(prec_);
+    if (bytes > d_sym_bytes_) {
+      (void) hipFree(d_sym_);
+      d_sym_ = nullptr;
+      d_sym_bytes_ = 0;
+      if (hipMalloc(&d_sym_, bytes) != hipSuccess)
+        throw Error(DREAM_HIP_ENOMEM, "das: device allocation failed (symmetrized A-scans)");
+      d_sym_bytes_ = bytes;
+    }
+    const dim3 grid((unsigned) Lt_, (unsigned) Lt_, (unsigned) n_frames);
+    if (prec_ == DREAM_HIP_F64)
+      hipLaunchKernelGGL((das_symmetrize_kernel<double, double>), grid, dim3(256), 0, s,
+                         (const double *) Y, (double *) d_sym_, (int) Lt_, K_, y_stride,
+                         (const double *) delay);
+    else if (prec_ == DREAM_HIP_F32)
+      hipLaunchKernelGGL((das_symmetrize_kernel<float, double>), grid, dim3(256), 0, s,
+                         (const float *) Y, (float *) d_sym_, (int) Lt_, K_, y_stride,
+                         (const double *) delay);
+    else
+      hipLaunchKernelGGL((das_symmetrize_kernel<float, float>), grid, dim3(256), 0, s,
+                         (const float *) Y, (float *) d_sym_, (int) Lt_, K_, y_stride,
+                         (const float *) delay);
+    DREAM_HIP_CHECK(hipGetLastError());
+    a.Ysym = d_sym_;
+    a.recip = 1;
+  }
+  if (timed) DREAM_HIP_CHECK(hipEventRecord(tev_[(size_t) 3 * t_n_ + 1], s));
   ChunkGate gate(s);
   if (No_ > 0) launch_das_forward(a, type_, prec_, interp, apod, s, &gate);
+  if (timed) DREAM_HIP_CHECK(hipEventRecord(tev_[(size_t) 3 * t_n_++ + 2], s));
   if (!want_status) return E_NONE;
   DREAM_HIP_CHECK(hipMemcpyAsync(h_status_, d_status_, sizeof(Status), hipMemcpyDeviceToHost, s));
   DREAM_HIP_CHECK(hipStreamSynchronize(s));
@@ -1645,10 +1823,14 @@ int DasHandle::run_host(const void *Y, const void *Ro, const void *Gt, const voi
   }
   put_geom(dRo, Ro, No_ * 3);
   put_geom(dGt, Gt, Lt_ * 3);
-  if (bGr) put_geom(dGr, Gr, Lr_ * 3);
+  // One array for both sides (equal bytes): pass one device copy for both,
+  // which lets a TFM pair (i, j) with (j, i).
+  const bool same = bGr && Lr_ == Lt_ &&
+                    std::memcmp(Gt, Gr, (size_t) Lt_ * 3 * in_gs) == 0;
+  if (bGr && !same) put_geom(dGr, Gr, Lr_ * 3);
   put_geom(dD, delay, nd);
-  int code = run_dev(dY, dRo, dGt, bGr ? dGr : nullptr, dt, delay_type, dD, cp, nullptr, nullptr,
-                     DREAM_HIP_NEAREST, 1, 0, 0, dIm, err_level, true, s);
+  int code = run_dev(dY, dRo, dGt, bGr ? (same ? dGt : dGr) : nullptr, dt, delay_type, dD, cp,
+                     nullptr, nullptr, DREAM_HIP_NEAREST, 1, 0, 0, dIm, err_level, true, s);
   DREAM_HIP_CHECK(hipMemcpyAsync(Im, dIm, bIm, hipMemcpyDeviceToHost, s));
   DREAM_HIP_CHECK(hipStreamSynchronize(s));
   return code;

@@ -36,6 +36,14 @@ struct DasArgs {
   // First block (forward tile, adjoint region or pixel-adjoint tile) and
   // first frame of this launch: a long call runs as several chunk launches.
   int64_t block0, frame0, block_end;
+  // TFM with one array for transmit and receive and a scalar delay: Ysym
+  // holds Y[i][j] + Y[j][i] at (i, j), i < j, and Y[i][i] at (i, i) (same
+  // layout and frame stride as Y).  The forward then visits the pairs i <= j
+  // only, whenever the delay read on the device is zero (the pair's sample
+  // index is then the same for (i, j) and (j, i): legs computed by the same
+  // expression, added commutatively).
+  const void *Ysym;
+  int32_t recip;
 };
 
 // Bytes of forward partial-sum scratch a launch of this shape uses.
@@ -99,13 +107,24 @@ private:
   size_t d_z_bytes_ = 0;
   void *d_scratch_ = nullptr;    // forward partial sums (split launches)
   size_t d_scratch_bytes_ = 0;
+  void *d_sym_ = nullptr;        // reciprocal TFM: symmetrized A-scans (Ysym)
+  size_t d_sym_bytes_ = 0;
   int32_t *d_perm_ = nullptr;    // pixel order (No entries) or null
 public:
   // Set (perm != null, n == No) or clear (perm == null) the pixel order; perm
   // is a host array, copied.
   void set_pixel_order(const int32_t *perm, int64_t n);
+  // Kernel timing: the next n_max forward calls record HIP events on their
+  // stream before the preparation kernel (the reciprocal symmetrization, if
+  // any), between it and the beamforming kernel, and after that; read_timing
+  // waits for them, sums both segments (ms) and stops.
+  void start_timing(int n_max);
+  void read_timing(int *n, double *prep_ms, double *forward_ms);
 private:
   void ensure_scratch(size_t bytes);
+  void drop_timing();
+  std::vector<hipEvent_t> tev_;  // 3 per timed call
+  int t_cap_ = 0, t_n_ = 0;
 };
 
 // fs_khz, inv_cp, delay_ms (0), cp and leg_max of a launch, with the

@@ -99,6 +99,11 @@ class DasPlan:
         if self.type != 0:
             self.Gr = _dev_tensor(torch.as_tensor(Gr, dtype=gd).reshape(-1, 3).t(), gd,
                                   self.device)
+            if self.type == 1 and self.Gr.shape == self.Gt.shape and torch.equal(self.Gr,
+                                                                                  self.Gt):
+                # One array for both sides: passing the same device array lets
+                # the library pair (i, j) with (j, i) (reciprocal TFM).
+                self.Gr = self.Gt
             self.Lr = self.Gr.shape[1]
         else:
             self.Gr = None
@@ -112,6 +117,18 @@ class DasPlan:
         self._delay_cache = {}
         if pixel_order:
             self.set_pixel_order(pixel_order_of(Ro))
+
+    def time_kernels(self, n_max):
+        """Record HIP events around the kernels of the next n_max forward calls."""
+        _lib.check(_lib.lib().dream_hip_das_time_kernels(self.h, int(n_max)))
+
+    def kernel_times(self):
+        """(timed calls, summed preparation ms, summed beamforming ms); stops timing."""
+        n = ctypes.c_int(0)
+        prep, fwd = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _lib.check(_lib.lib().dream_hip_das_kernel_times(self.h, ctypes.byref(n),
+                                                         ctypes.byref(prep), ctypes.byref(fwd)))
+        return n.value, prep.value, fwd.value
 
     def set_pixel_order(self, perm):
         """Visit the pixels in `perm` order (None: 0..No-1)."""

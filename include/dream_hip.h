@@ -366,6 +366,15 @@ int dream_hip_das_adjoint_dev(void *handle, const void *Y, const void *Ro, const
                               int64_t im_frame_stride, const void *g, void *gY,
                               void *g_apod_t, void *g_apod_r, void *g_cp, void *stream);
 
+/* Measurement hook, no counterpart in the reference: the next n_max forward
+ * calls (dream_hip_das_dev / dream_hip_das) of the handle record HIP events
+ * on their stream around their kernels; dream_hip_das_kernel_times waits for
+ * them and returns the number of timed calls and the summed durations (ms)
+ * of the preparation kernel (the symmetrization of a reciprocal TFM call, 0
+ * otherwise) and of the beamforming kernel(s), then stops timing. */
+int dream_hip_das_time_kernels(void *handle, int n_max);
+int dream_hip_das_kernel_times(void *handle, int *n, double *prep_ms, double *forward_ms);
+
 /* dream_hip_das_adjoint_dev that allocates nothing.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under
  * hipGraph capture, tight serving loops).  Same arguments and results; the

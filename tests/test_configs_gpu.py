@@ -369,6 +369,20 @@ def test_chunked_calls_equal_single_launches(gpu, chunking):
             if x is not None:
                 assert rel_l2(x.cpu().numpy(), y.cpu().numpy()) <= 1e-12, work
         chunking(0, 0)
+    # A small image (heuristic shape: split partial sums) in chunks of 3 tiles,
+    # each chunk reduced on its own: the same fixed-order sums.
+    # (64 elements: 16 jobs per tile, so the heuristic splits each tile in two.)
+    G, Ro, Y = _tfm_case(No=3000, L=64, K=512)
+    Yt = torch.tensor(Y, device="cuda")
+    gpu.dream_hip_set_autotune(0)
+    try:
+        small = DasPlan("tfm", Y.shape[1], Ro, G, G, precision="f64")
+        one = small.forward(Yt, 0.02, 0.1, 1500.0)
+        chunking(3 * 64 * 4096, 0)
+        assert torch.equal(small.forward(Yt, 0.02, 0.1, 1500.0), one)
+        chunking(0, 0)
+    finally:
+        gpu.dream_hip_set_autotune(1)
     # SIR plan: items in chunks.
     import workloads
     probs = workloads.sir_small_uneven_batch(n_problems=12)
@@ -410,6 +424,24 @@ def test_cancel_between_chunks(gpu, chunking):
     chunking(0, 0)
     gpu.dream_hip_set_running()
     assert torch.equal(plan.forward(Yt, 0.02, 0.1, 1500.0), full)
+    # The same with split partial sums (a small image, heuristic shape).
+    Gs, Rs, Ys = _tfm_case(No=3000, L=64, K=512)
+    Yst = torch.tensor(Ys, device="cuda")
+    gpu.dream_hip_set_autotune(0)
+    try:
+        small = DasPlan("tfm", Ys.shape[1], Rs, Gs, Gs, precision="f64")
+        sfull = small.forward(Yst, 0.02, 0.1, 1500.0)
+        chunking(3 * 64 * 4096, 2)
+        sout = torch.full((Rs.shape[0],), float("nan"), dtype=torch.float64, device="cuda")
+        with pytest.raises(_lib.Cancelled):
+            small.forward(Yst, 0.02, 0.1, 1500.0, out=sout)
+        torch.cuda.synchronize()
+        sdone = ~torch.isnan(sout)
+        assert int(sdone.sum()) == 2 * 3 * 64 and torch.equal(sout[sdone], sfull[sdone])
+    finally:
+        gpu.dream_hip_set_autotune(1)
+        chunking(0, 0)
+        gpu.dream_hip_set_running()
     # The adjoint and a SIR plan stop the same way.
     g = torch.ones(Ro.shape[0], dtype=torch.float64, device="cuda")
     chunking(TILE_WORK * tiles_per_chunk, 1)

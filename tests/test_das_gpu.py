@@ -446,3 +446,55 @@ def test_autotuned_shapes_give_the_same_image(gpu):
         assert torch.equal(first, off) and torch.equal(again, off)
     finally:
         _lib.lib().dream_hip_set_autotune(1)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32_ref"])
+def test_reciprocal_tfm_equals_full_pair_sum(gpu, oracle, prec):
+    """One array on both sides and a zero delay: the forward sums each pair
+    (i, j), (j, i) once over the symmetrized data (same sample index, the
+    legs being computed by the same expression).  Against the full pair sum
+    (the receive array passed as a separate copy): bit for bit on ramp data,
+    to rounding on random data, linear interpolation, frames, split launches
+    and the out-of-bounds status; a non-zero delay reads Y itself."""
+    from dream_amd.torch_ops import DasPlan
+    dt_ = torch.float64 if prec == "f64" else torch.float32
+    for L, No in ((64, 3000), (24, 20000)):              # split launch; unsplit, ragged blocks
+        rng = np.random.default_rng(81 + L)
+        K = 1200                                          # some pairs fall past K
+        x = np.linspace(-8.0, 8.0, L)
+        G = np.c_[x, np.zeros(L), np.zeros(L)]
+        Ro = np.c_[rng.uniform(-10, 10, No), np.zeros(No), rng.uniform(1, 12, No)]
+        rec = DasPlan("tfm", K, Ro, G, G, precision=prec)
+        assert rec.Gr is rec.Gt
+        full = DasPlan("tfm", K, Ro, G, G, precision=prec)
+        full.Gr = full.Gt.clone()
+        ramp = torch.arange(K, dtype=dt_, device="cuda").remainder(512).expand(L * L, K)
+        Yr = torch.stack([ramp, 2 * ramp]).contiguous()
+        a = rec.forward(Yr, 0.02, 0.0, 1500.0)
+        b = full.forward(Yr, 0.02, 0.0, 1500.0)
+        assert torch.equal(a, b), (L, prec)
+        Yn = torch.tensor(rng.standard_normal((2, L * L, K)), dtype=dt_, device="cuda")
+        for interp in ("nearest", "linear"):
+            a = rec.forward(Yn, 0.02, 0.0, 1500.0, interp=interp).double().cpu().numpy()
+            b = full.forward(Yn, 0.02, 0.0, 1500.0, interp=interp).double().cpu().numpy()
+            assert rel_l2(a, b) <= (1e-13 if prec == "f64" else 1e-6), (L, prec, interp)
+        a, ca = rec.forward(Yn[0].contiguous(), 0.02, 0.0, 1500.0, err_level="warn",
+                            want_status=True)
+        b, cb = full.forward(Yn[0].contiguous(), 0.02, 0.0, 1500.0, err_level="warn",
+                             want_status=True)
+        assert ca == cb == 2
+        # Non-zero delay: the device-side test sends the forward to Y (every
+        # pair gathered; the launch shape, tuned for the paired job list, may
+        # sum in another order).
+        a = rec.forward(Yr, 0.02, 0.3, 1500.0)
+        b = full.forward(Yr, 0.02, 0.3, 1500.0)
+        assert torch.equal(a, b), (L, prec)
+        a = rec.forward(Yn, 0.02, 0.3, 1500.0).double().cpu().numpy()
+        b = full.forward(Yn, 0.02, 0.3, 1500.0).double().cpu().numpy()
+        assert rel_l2(a, b) <= (1e-13 if prec == "f64" else 1e-6), (L, prec)
+    # And against the oracle on the C2-like case through the host path.
+    Y, Gt, Gr, Ro, rng = _setup("tfm", K=800, L=16, No=900, seed=12)
+    ref, err = oracle.das(Y, Gt, Gt, Ro, 0.02, 0.0, 1500.0, "tfm", "warn", "f64", nthreads=4)
+    from dream_amd import das
+    Im = das.das(Y, Gt, Gt.copy(), Ro, 0.02, 0.0, 1500.0, "tfm", "warn", "gpu")
+    assert np.max(np.abs(Im - ref)) <= 1e-13 * np.abs(Y).max() * Y.shape[1]
