@@ -276,6 +276,29 @@ def _subgrid(w, d):
     return out
 
 
+def bench_c1(steps, warmup):
+    """C1 alone (FP64 SIR, one problem): the plan replayed as a captured graph
+    and launched eagerly, and through dream_arr_rect_dev (plan set-up
+    included), per call."""
+    import torch
+
+    import workloads
+    from dream_amd.torch_ops import SirBatch, arr_rect
+    w = workloads.c1()
+    q = _sir_device_problem(w, torch)
+    batch = SirBatch([q], out_dtype=torch.float64)
+    out = {}
+    for name, g in (("graph", True), ("eager", False)):
+        ms = time_launches(lambda: batch.run(use_graph=g), steps, warmup)
+        out[f"c1_plan_{name}"] = dict(ms=round(ms, 4))
+    H = torch.empty((w["Ro"].shape[0], w["nt"]), dtype=torch.float64, device="cuda")
+    ms = time_launches(lambda: arr_rect(q["Ro"], w["a"], w["b"], w["dx"], w["dy"], w["dt"],
+                                        w["nt"], q["delay"], w["v"], w["cp"], w["G"],
+                                        err_level="stop", out=H), max(2, steps // 4), 2)
+    out["c1_arr_rect_dev_call"] = dict(ms=round(ms, 4))
+    return out
+
+
 def bench_sir(steps, warmup):
     """C1 (FP64, one problem) and C3 (FP32, 32 uneven problems, one persistent launch
     under a captured graph, against one dream_arr_rect_dev call per problem)."""
@@ -288,10 +311,14 @@ def bench_sir(steps, warmup):
     # C1
     w = workloads.c1()
     batch = SirBatch([_sir_device_problem(w, torch)], out_dtype=torch.float64)
-    ms = time_launches(lambda: batch.run(use_graph=True), steps, warmup)
+    # A launch of two small kernels (status and output zeroing, the SIR sum):
+    # eager launches beat a graph replay here.
+    ms = time_launches(lambda: batch.run(use_graph=False), steps, warmup)
+    ms_g = time_launches(lambda: batch.run(use_graph=True), steps, warmup)
     n_oe = w["Ro"].shape[0] * w["G"].shape[0]
     contrib = _sir_contributions(w)
-    out["c1_sir_f64"] = dict(ms=round(ms, 4), obs_elem_per_s=n_oe / (ms * 1e-3),
+    out["c1_sir_f64"] = dict(ms=round(ms, 4), graph_ms=round(ms_g, 4),
+                             obs_elem_per_s=n_oe / (ms * 1e-3),
                              contributions_per_s=contrib / (ms * 1e-3),
                              roofline_frac=(contrib * 16 + w["Ro"].shape[0] * w["nt"] * 8)
                              / (ms * 1e-3) / HBM_PEAK)
@@ -660,8 +687,9 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: headline kernel only, plus a 256 MiB "
                          "device-to-device copy used to calibrate the HBM byte counters")
-    ap.add_argument("--part", choices=("sir", "c4", "c5", "batch", "conv"),
+    ap.add_argument("--part", choices=("c1", "sir", "c4", "c5", "batch", "conv"),
                     help="profile one of the other paths alone (short run, no JSON line): "
+                         "c1 = C1 SIR alone, "
                          "sir = C1 + C3 SIR, c4 = SAFT forward/adjoint, c5 = RCA frames, "
                          "batch = uneven batches of small DAS and SIR problems, "
                          "conv = data-synthesis convolution")
@@ -678,7 +706,7 @@ def main():
     from dream_amd import _lib
     _lib.require_device()
     if args.part:
-        part = {"sir": bench_sir, "c4": bench_das_diff_c4, "c5": bench_das_diff_c5,
+        part = {"c1": bench_c1, "sir": bench_sir, "c4": bench_das_diff_c4, "c5": bench_das_diff_c5,
                 "batch": bench_batches, "conv": bench_conv}[args.part]
         log(json.dumps(part(args.steps, args.warmup)))
         return

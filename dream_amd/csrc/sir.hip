@@ -283,14 +283,6 @@ sir_kernel(const SirDev *__restrict__ probs, int n_probs, int64_t item_lo, int64
   }
 }
 
-// Zero `n` elements of `bytes_per`-byte words (graph-safe replacement for a
-// memset node).
-__global__ void fill_zero_kernel(unsigned int *p, int64_t nwords)
-{
-  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nwords;
-       i += (int64_t) gridDim.x * blockDim.x)
-    p[i] = 0u;
-}
 
 __global__ void rect_indices_kernel(double xo, double yo, double zo, const double *xs, int nx,
                                     const double *ys, int ny, double cp, double dt,
@@ -339,14 +331,47 @@ int device_cus()
   return cus;
 }
 
-void launch_fill_zero(void *p, size_t bytes, hipStream_t s)
+// Up to kZeroRanges buffers zeroed by one launch (graph-safe; a plan's
+// status block and the outputs it accumulates into): blockIdx.y picks the
+// buffer, blockIdx.x strides over its 32-bit words.
+constexpr int kZeroRanges = 16;
+struct ZeroList {
+  unsigned int *p[kZeroRanges];
+  int64_t words[kZeroRanges];
+};
+
+__global__ void fill_zero_list_kernel(ZeroList z)
 {
-  if (!bytes) return;
-  const int64_t words = (int64_t) (bytes / 4);
-  const int blocks = (int) std::min<int64_t>((words + 255) / 256, 4096);
-  fill_zero_kernel<<<blocks, 256, 0, s>>>(reinterpret_cast<unsigned int *>(p), words);
-  DREAM_HIP_CHECK(hipGetLastError());
+  unsigned int *p = z.p[blockIdx.y];
+  const int64_t n = z.words[blockIdx.y];
+  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t) gridDim.x * blockDim.x)
+    p[i] = 0u;
 }
+
+// Zero the buffers (pointer, bytes; bytes a multiple of 4) in as few
+// launches as possible.
+void launch_fill_zero_list(const std::vector<std::pair<void *, size_t>> &bufs, hipStream_t s)
+{
+  size_t k = 0;
+  while (k < bufs.size()) {
+    ZeroList z;
+    int n = 0;
+    int64_t most = 0;
+    for (; k < bufs.size() && n < kZeroRanges; k++) {
+      if (!bufs[k].second) continue;
+      z.p[n] = reinterpret_cast<unsigned int *>(bufs[k].first);
+      z.words[n] = (int64_t) (bufs[k].second / 4);
+      most = std::max(most, z.words[n]);
+      n++;
+    }
+    if (!n) break;
+    const unsigned bx = (unsigned) std::min<int64_t>((most + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_zero_list_kernel, dim3(bx, (unsigned) n), dim3(256), 0, s, z);
+    DREAM_HIP_CHECK(hipGetLastError());
+  }
+}
+
 
 } // namespace
 
@@ -776,12 +801,13 @@ void SirPlan::launch(hipStream_t s, ChunkGate *gate)
 
 void SirPlan::launch_shape(const SirShape &sh, hipStream_t s, ChunkGate *gate)
 {
-  launch_fill_zero(d_status_, host_.size() * sizeof(Status), s);
+  // The status blocks and the outputs that items accumulate into, zeroed
+  // by one launch (usually).
+  std::vector<std::pair<void *, size_t>> zero{{d_status_, host_.size() * sizeof(Status)}};
+  const size_t esz = out_dtype_ == DREAM_HIP_F64 ? 8 : 4;
   for (size_t i = 0; i < zero_len_.size(); i++)
-    if (zero_len_[i]) {
-      const size_t esz = out_dtype_ == DREAM_HIP_F64 ? 8 : 4;
-      launch_fill_zero(zero_ptr_[i], (size_t) zero_len_[i] * esz, s);
-    }
+    if (zero_len_[i]) zero.emplace_back(zero_ptr_[i], (size_t) zero_len_[i] * esz);
+  launch_fill_zero_list(zero, s);
   if (n_items_ == 0) return;
   unsigned long long *ctr = sh.persistent ? d_counter_ : nullptr;
   bool any_global = false;
