@@ -3,8 +3,9 @@ inside the DR sums) pinned by limits and by an independent FFT.  FFTW (the
 reference's transform) is absent here and the reference ships no golden
 vectors for attenuation; the transform convention (complex-to-real inverse,
 divided by nt) and the magnitude law are pinned through numpy's FFT below.
-The dispersion phase -w r a b/pi^2, b = log(1/(a1 w)), is pinned only by
-reading src/attenuation.cc:100-112 (parity unpinned for that term)."""
+The dispersion phase -w r a b/pi^2, b = log(1/(a1 w)), is pinned by the
+Kramers-Kronig (minimum-phase) relation to the magnitude law, measured on the
+oracle's own samples."""
 import numpy as np
 
 from oracle import oracle as O
@@ -94,3 +95,32 @@ def test_attenuation_reaches_discs_cylinders_and_annuli():
     Ha, _, _ = O.dream_arr_annu(Ro, [0.4, 0.5, 0.8], 0.1, 0.1, DT, 128, 5.0, 1.0, CP,
                                 nthreads=2, alpha=5.0)
     assert np.array_equal(Ha[1:], (D[0.4] + (D[0.8] - D[0.5]))[:-1])
+
+
+def test_dispersion_phase_is_the_minimum_phase_of_linear_absorption():
+    """Kramers-Kronig for absorption linear in frequency (Futterman 1962; Aki
+    and Richards 1980, ch. 5): a causal, minimum-phase filter with
+    ln|X(w)| = -c w has the phase (2 c / pi) w ln(w / w0) up to a pure delay.
+    Measured on one attenuated sub-element's spectrum (numpy's FFT of the
+    oracle's samples, the t w delay removed): the magnitude decay c and the
+    slope s of phase/w against ln w must satisfy s = (2/pi) c.  A dropped
+    pi, a sign error or a missing dispersion term fails.  The reference
+    frequency w0 = pi / (0.95 dt) is the reference's causality constant
+    (src/attenuation.cc:96-98)."""
+    for nt, alpha, zo in ((257, 3.0, 20.0), (401, 8.0, 35.0)):
+        h, r, ai, it, e = _one_sub_element(0.0, 0.0, zo, zo * 1000.0 / CP - 5.0, nt, alpha)
+        assert e == 0
+        X = np.fft.rfft(h) / ai
+        k = np.arange(1, X.size)
+        w = 2 * np.pi * k / (nt * DT * 1e-6)                 # rad/s
+        t = it * DT * 1e-6                                    # s
+        Xk = X[1:] * np.exp(1j * t * w)                       # remove the sample delay
+        phase = np.unwrap(np.angle(Xk))
+        keep = np.abs(Xk) > 1e-6                              # above the FFT's rounding
+        assert keep.sum() > 40
+        c = -np.polyfit(w[keep], np.log(np.abs(Xk[keep])), 1)[0]        # ln|X| = -c w
+        s, icpt = np.polyfit(np.log(w[keep]), phase[keep] / w[keep], 1)  # phase/w = s ln w + b
+        assert abs(s - 2.0 * c / np.pi) <= 1e-6 * abs(s)
+        assert s > 0
+        a1 = DT * 1e-6 * 0.95 / np.pi
+        assert abs(icpt - s * np.log(a1)) <= 1e-6 * abs(s * np.log(a1))
