@@ -32,6 +32,7 @@ EXPORTS = (
     "dream_hip_test_chunking",
     "dream_hip_arr_rect", "dream_hip_rect", "dream_hip_arr_rect_dev",
     "dream_hip_rect_indices", "dream_hip_rsq_seed",
+    "dream_hip_sir_att_dev",
     "dream_hip_sir_plan_create", "dream_hip_sir_plan_run",
     "dream_hip_sir_plan_status", "dream_hip_sir_plan_destroy",
     "dream_hip_das_create", "dream_hip_das_destroy", "dream_hip_das_setup_has_changed",
@@ -149,6 +150,9 @@ def _declare(L):
     L.dream_hip_rect_indices.argtypes = [_dbl] * 10 + [_i64p, _i64, _i64p]
     L.dream_hip_rsq_seed.restype = _i32
     L.dream_hip_rsq_seed.argtypes = [_vp, _i64, _vp]
+    L.dream_hip_sir_att_dev.restype = _i32
+    L.dream_hip_sir_att_dev.argtypes = [ctypes.POINTER(DreamSirProblem), ctypes.c_int32, _dbl,
+                                        _i32, _vp, _vp, _vp]
     L.dream_hip_sir_plan_create.restype = _i32
     L.dream_hip_sir_plan_create.argtypes = [ctypes.POINTER(DreamSirProblem), ctypes.c_int32, _i32,
                                             ctypes.POINTER(_vp)]

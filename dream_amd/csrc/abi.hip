@@ -500,6 +500,45 @@ int dream_hip_rsq_seed(const double *s, int64_t n, double *y)
   });
 }
 
+namespace {
+std::vector<SirProblem> sir_problems(const DreamSirProblem *problems, int32_t n)
+{
+  std::vector<SirProblem> probs;
+  probs.reserve((size_t) n);
+  for (int32_t i = 0; i < n; i++) {
+    const DreamSirProblem &q = problems[i];
+    require(q.shape == SIR_RECT || q.shape == SIR_CIRC || q.shape == SIR_CYLIND,
+            "SIR: unknown shape");
+    probs.push_back(make_problem(q.Ro, q.No, q.a, q.b, q.dx, q.dy, q.dt, q.nt, q.delay_type,
+                                 q.delay, q.v, q.cp, q.num_elements, q.G, q.foc_met, q.focal,
+                                 q.steer_met, q.theta, q.phi, q.apod, q.do_apod, q.apod_met,
+                                 q.apod_par, q.h, q.err_level));
+    probs.back().shape = q.shape;
+    probs.back().R = q.R;
+  }
+  return probs;
+}
+} // namespace
+
+int dream_hip_sir_att_dev(const DreamSirProblem *problems, int32_t n, double alpha,
+                          int out_dtype, int32_t *sir_err, int64_t *oob_count, void *stream)
+{
+  return guarded([&]() {
+    require((problems || n == 0) && n >= 0, "sir_att_dev: bad arguments");
+    if (cancelled()) throw Error(DREAM_HIP_ECANCELLED, "SIR cancelled");
+    hipStream_t s = (hipStream_t) stream;   // NULL: the default stream
+    const std::vector<SirProblem> probs = sir_problems(problems, n);
+    if (alpha > 2.220446049250313e-16) {          // DBL_EPSILON
+      run_sir_att(probs, alpha, out_dtype, s, sir_err, oob_count);
+    } else {
+      SirPlan plan(probs, out_dtype);
+      plan.run(s, false);
+      plan.status(sir_err, oob_count);
+    }
+    return DREAM_HIP_OK;
+  });
+}
+
 int dream_hip_sir_plan_create(const DreamSirProblem *problems, int32_t n, int out_dtype,
                               void **plan)
 {

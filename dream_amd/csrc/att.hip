@@ -18,6 +18,8 @@
 // complex-to-real sum (the imaginary parts of X[0] and X[nt/2] drop out, as
 // in FFTW's c2r transform).  Cost: nt/2 complex exponentials per
 // contribution, as the reference pays, without its per-contribution FFT.
+// A work-group holds at most kBinBlock bins; longer responses split each
+// point's bins over several work-groups (any nt).
 
 #include "sir.hpp"
 
@@ -47,9 +49,15 @@ struct AttDev {
   double cp, dt, dt_s, alpha;   // alpha in Np/(m Hz)
   double *spec;         // No x K x 2 (re, im), zeroed before the launch
   void *h;  int32_t out_f64;  int32_t chunks;
+  int32_t bin_blocks;   // the K bins in blocks of kBinBlock, one work-group each
   Status *status;
   int64_t item_begin;
 };
+
+// Spectrum bins one work-group accumulates in LDS (two FP64 accumulators
+// each, beside the 256-contribution staging): nt up to 2 (kBinBlock - 1) in
+// one block, longer ones over several.
+constexpr int64_t kBinBlock = 3584;
 
 __device__ __forceinline__ int find_att_problem(const AttDev *P, int n, int64_t item)
 {
@@ -67,7 +75,8 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v)
   return v;
 }
 
-// Spectrum of one (point, chunk): sum over its contributions of ai * X[k].
+// Spectrum bins [k0, k1) of one (point, chunk): sum over the chunk's
+// contributions of ai * X[k].  A work item is (point, chunk, bin block).
 __global__ void __launch_bounds__(kThreads)
 att_spectrum_kernel(const AttDev *__restrict__ probs, int n_probs)
 {
@@ -76,18 +85,23 @@ att_spectrum_kernel(const AttDev *__restrict__ probs, int n_probs)
   const int pi = find_att_problem(probs, n_probs, item);
   const AttDev &P = probs[pi];
   const int64_t local = item - P.item_begin;
-  const int64_t n = local / P.chunks;
-  const int c = (int) (local - n * P.chunks);
+  const int64_t per_point = (int64_t) P.chunks * P.bin_blocks;
+  const int64_t n = local / per_point;
+  const int64_t rest = local - n * per_point;
+  const int c = (int) (rest / P.bin_blocks);
+  const int bb = (int) (rest - (int64_t) c * P.bin_blocks);
   const int64_t K = P.nt / 2 + 1;
+  const int64_t k0 = (int64_t) bb * kBinBlock, k1 = std::min<int64_t>(K, k0 + kBinBlock);
+  const int64_t nk = k1 - k0;
   const int64_t total = (int64_t) P.L * P.S;
   const int64_t c0 = (int64_t) c * total / P.chunks, c1 = (int64_t) (c + 1) * total / P.chunks;
 
-  double *acc_re = reinterpret_cast<double *>(smem);        // K
-  double *acc_im = acc_re + K;                              // K
-  double *st_r = acc_im + K;                                // kThreads: r [m]
+  double *acc_re = reinterpret_cast<double *>(smem);        // nk
+  double *acc_im = acc_re + nk;                             // nk
+  double *st_r = acc_im + nk;                               // kThreads: r [m]
   double *st_t = st_r + kThreads;                           //           it * dt [s]
   double *st_a = st_t + kThreads;                           //           ai (0: skipped)
-  for (int64_t k = threadIdx.x; k < K; k += kThreads) { acc_re[k] = 0.0; acc_im[k] = 0.0; }
+  for (int64_t q = threadIdx.x; q < nk; q += kThreads) { acc_re[q] = 0.0; acc_im[q] = 0.0; }
 
   gdouble *Ro = (gdouble *) P.Ro;
   const double xo = Ro[n], yo = Ro[n + P.No], zo = Ro[n + 2 * P.No];
@@ -131,8 +145,9 @@ att_spectrum_kernel(const AttDev *__restrict__ probs, int n_probs)
       }
     }
     __syncthreads();
-    for (int64_t k = threadIdx.x; k < K; k += kThreads) {
-      double re = acc_re[k], im = acc_im[k];
+    for (int64_t q = threadIdx.x; q < nk; q += kThreads) {
+      const int64_t k = k0 + q;
+      double re = acc_re[q], im = acc_im[q];
       if (k == 0) {
         for (int j = 0; j < nb; j++) re += st_a[j];     // X[0] = 1
       } else {
@@ -149,22 +164,24 @@ att_spectrum_kernel(const AttDev *__restrict__ probs, int n_probs)
           im += ai * (x1 * sn);
         }
       }
-      acc_re[k] = re;
-      acc_im[k] = im;
+      acc_re[q] = re;
+      acc_im[q] = im;
     }
   }
   __syncthreads();
   double *spec = P.spec + n * K * 2;
-  for (int64_t k = threadIdx.x; k < K; k += kThreads) {
+  for (int64_t q = threadIdx.x; q < nk; q += kThreads) {
+    const int64_t k = k0 + q;
     if (P.chunks == 1) {
-      spec[2 * k] = acc_re[k];
-      spec[2 * k + 1] = acc_im[k];
+      spec[2 * k] = acc_re[q];
+      spec[2 * k + 1] = acc_im[q];
     } else {
-      unsafeAtomicAdd(spec + 2 * k, acc_re[k]);
-      unsafeAtomicAdd(spec + 2 * k + 1, acc_im[k]);
+      unsafeAtomicAdd(spec + 2 * k, acc_re[q]);
+      unsafeAtomicAdd(spec + 2 * k + 1, acc_im[q]);
     }
   }
-  const unsigned wsum = wave_sum_u(oob);
+  // Every bin block sees the same contributions: the first one counts them.
+  const unsigned wsum = bb == 0 ? wave_sum_u(oob) : 0u;
   const bool any_last = __any(oob_last);
   if ((threadIdx.x & 63) == 0 && wsum) {
     atomicOr(&P.status->any_oob, 1u);
@@ -321,17 +338,16 @@ void run_sir_att(const std::vector<SirProblem> &problems, double alpha_db, int o
     chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, (total + kThreads - 1) / kThreads));
     require(chunks < (1ll << 31), "SIR: too many chunks");
     d.chunks = (int32_t) chunks;
+    d.bin_blocks = (int32_t) ((K + kBinBlock - 1) / kBinBlock);
     d.item_begin = items;
-    items += (S > 0) ? p.No * chunks : 0;
+    items += (S > 0) ? p.No * chunks * d.bin_blocks : 0;
     spec_off[i] = spec_total;
     spec_total += p.No * K * 2;
     point_begin[i] = points;
     points += p.No;
-    max_K = std::max<size_t>(max_K, (size_t) K);
+    max_K = std::max<size_t>(max_K, (size_t) std::min<int64_t>(K, kBinBlock));
   }
   const size_t lds = (2 * max_K + 3 * kThreads) * sizeof(double);
-  require(lds <= 64 * 1024, "SIR with attenuation: nt above 7422 is not supported "
-                            "(one work-group holds a point's spectrum in LDS)");
 
   double *d_tables = nullptr, *d_spec = nullptr;
   AttDev *d_probs = nullptr;

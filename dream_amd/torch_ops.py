@@ -394,6 +394,57 @@ def arr_annu(Ro, Gr, dx, dy, dt, nt, delay, v, cp, foc_met="off", focal=0.0,
     return H, code.value, oob.value
 
 
+def _fill_sir_problem(q, p, H):
+    """Fill DreamSirProblem q from the problem dict p (SirBatch's keys) with
+    output H; returns the objects whose memory q points into."""
+    Ro = p["Ro"]
+    No = _check_points(Ro)
+    G = np.asfortranarray(np.asarray(p["G"], dtype=np.float64).reshape(-1, 3))
+    L = G.shape[0]
+    d, dtype = _sir_delay(p.get("delay", 0.0), Ro, No)
+    focal = _host_f64(np.atleast_1d(p.get("focal", 0.0)))
+    apod = _host_f64(p["apod"] if p.get("apod") is not None else np.ones(L))
+    _host_lengths(p.get("foc_met", "off"), focal, p.get("apod_met", "off"), apod, L)
+    _check(H, "out", Ro.device, numel=No * int(p["nt"]))
+    q.Ro, q.No = Ro.data_ptr(), No
+    q.a, q.b = p.get("a", 0.0), p.get("b", 0.0)
+    q.dx, q.dy, q.dt, q.nt = p["dx"], p["dy"], p["dt"], int(p["nt"])
+    q.delay_type, q.delay = dtype, d.data_ptr()
+    q.v, q.cp = p.get("v", 1.0), p["cp"]
+    q.num_elements, q.G = L, G.ctypes.data
+    q.foc_met, q.focal = _lib.FOC_MET[p.get("foc_met", "off")], focal.ctypes.data
+    q.steer_met = _lib.STEER_MET[p.get("steer_met", "off")]
+    q.theta, q.phi = p.get("theta", 0.0), p.get("phi", 0.0)
+    apod_met = p.get("apod_met", "off")
+    q.apod, q.do_apod = apod.ctypes.data, 0 if apod_met == "off" else 1
+    q.apod_met, q.apod_par = _lib.APOD_MET.get(apod_met, 0), p.get("apod_par", 0.0)
+    q.err_level = _lib.ERR_LEVEL[p.get("err_level", "ignore")]
+    q.h = H.data_ptr()
+    q.shape = {"rect": 0, "circ": 1, "cylind": 2}[p.get("shape", "rect")]
+    q.R = float(p.get("Rcurv", 0.0) if q.shape == 2 else p.get("R", 0.0))
+    return [Ro, G, d, focal, apod, H]
+
+
+def sir_attenuated(problem, alpha, out_dtype=torch.float64, out=None):
+    """The SIR of one problem (SirBatch's dict keys: rectangles, discs or
+    cylindrically curved elements) with frequency-dependent attenuation
+    alpha [dB/(cm MHz)] on device tensors.  Returns (H (No, nt), SIRError
+    code, out-of-bounds contribution count)."""
+    _lib.require_device()
+    Ro = problem["Ro"]
+    No = _check_points(Ro)
+    H = _sir_out(out, No, problem["nt"], out_dtype, Ro.device)
+    q = (_lib.DreamSirProblem * 1)()
+    keep = _fill_sir_problem(q[0], problem, H)
+    code = ctypes.c_int32(0)
+    oob = ctypes.c_int64(0)
+    _lib.check(_lib.lib().dream_hip_sir_att_dev(
+        q, 1, float(alpha), _lib.F64 if out_dtype == torch.float64 else _lib.F32,
+        ctypes.byref(code), ctypes.byref(oob), _stream()))
+    del keep
+    return H, code.value, oob.value
+
+
 class SirBatch:
     """Many uneven dream_arr_rect problems in one persistent launch.
 
@@ -414,34 +465,10 @@ class SirBatch:
         self.H = []
         arr = (_lib.DreamSirProblem * max(self.n, 1))()
         for i, p in enumerate(problems):
-            Ro = p["Ro"]
-            No = _check_points(Ro)
-            G = np.asfortranarray(np.asarray(p["G"], dtype=np.float64).reshape(-1, 3))
-            L = G.shape[0]
-            d, dtype = _sir_delay(p.get("delay", 0.0), Ro, No)
-            focal = _host_f64(np.atleast_1d(p.get("focal", 0.0)))
-            apod = _host_f64(p["apod"] if p.get("apod") is not None else np.ones(L))
-            _host_lengths(p.get("foc_met", "off"), focal, p.get("apod_met", "off"), apod, L)
-            H = torch.zeros((No, int(p["nt"])), dtype=out_dtype, device=Ro.device)
+            H = torch.zeros((_check_points(p["Ro"]), int(p["nt"])), dtype=out_dtype,
+                            device=p["Ro"].device)
             self.H.append(H)
-            self._keep += [Ro, G, d, focal, apod, H]
-            q = arr[i]
-            q.Ro, q.No = Ro.data_ptr(), No
-            q.a, q.b = p.get("a", 0.0), p.get("b", 0.0)
-            q.dx, q.dy, q.dt, q.nt = p["dx"], p["dy"], p["dt"], int(p["nt"])
-            q.delay_type, q.delay = dtype, d.data_ptr()
-            q.v, q.cp = p.get("v", 1.0), p["cp"]
-            q.num_elements, q.G = L, G.ctypes.data
-            q.foc_met, q.focal = _lib.FOC_MET[p.get("foc_met", "off")], focal.ctypes.data
-            q.steer_met = _lib.STEER_MET[p.get("steer_met", "off")]
-            q.theta, q.phi = p.get("theta", 0.0), p.get("phi", 0.0)
-            apod_met = p.get("apod_met", "off")
-            q.apod, q.do_apod = apod.ctypes.data, 0 if apod_met == "off" else 1
-            q.apod_met, q.apod_par = _lib.APOD_MET.get(apod_met, 0), p.get("apod_par", 0.0)
-            q.err_level = _lib.ERR_LEVEL[p.get("err_level", "ignore")]
-            q.h = H.data_ptr()
-            q.shape = {"rect": 0, "circ": 1, "cylind": 2}[p.get("shape", "rect")]
-            q.R = float(p.get("Rcurv", 0.0) if q.shape == 2 else p.get("R", 0.0))
+            self._keep += _fill_sir_problem(arr[i], p, H)
         self.plan = ctypes.c_void_p(None)
         _lib.check(_lib.lib().dream_hip_sir_plan_create(arr, self.n,
                                                         _lib.F64 if out_dtype == torch.float64

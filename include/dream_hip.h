@@ -118,7 +118,7 @@ void dream_hip_test_chunking(int64_t work_per_chunk, int64_t cancel_after_chunks
  * reference's frequency-dependent attenuation (Attenuation::att,
  * src/attenuation.cc:72-140): every in-range contribution becomes a lossy
  * pulse, computed per point in the frequency domain and transformed once
- * (nt at most 7422).  Host pointers.  h (nt x No FP64)
+ * (any nt).  Host pointers.  h (nt x No FP64)
  * is overwritten (zeroed, then accumulated), which equals the reference's
  * accumulate-into-a-zeroed-buffer contract.
  * *sir_err: for err_level stop, out_of_bounds if any contribution fell
@@ -280,6 +280,17 @@ typedef struct DreamSirProblem {
                                                 2 cylinder (a x b, curvature radius R) */
   double R;                                  /* disc radius / curvature radius [mm] */
 } DreamSirProblem;
+
+/* The SIR of several problems (rectangles, discs, cylindrically curved
+ * elements; DreamSirProblem as for plans: Ro, delay and h DEVICE pointers)
+ * with the reference's frequency-dependent attenuation when alpha
+ * [dB/(cm MHz)] > DBL_EPSILON (Attenuation::att, src/attenuation.cc:72-140;
+ * any nt), or without it (then as a one-shot plan).  The device-pointer
+ * form of the attenuated dream_arr_rect / dream_arr_circ /
+ * dream_arr_cylind; per-problem SIRError codes and out-of-bounds counts
+ * (host arrays of n, may be NULL).  Synchronises `stream`. */
+int dream_hip_sir_att_dev(const DreamSirProblem *problems, int32_t n, double alpha,
+                          int out_dtype, int32_t *sir_err, int64_t *oob_count, void *stream);
 
 /* Build a plan (host preparation, device tables, LPT-ordered work list) for
  * n problems; the plan owns copies of every host-side table it needs. */
