@@ -468,7 +468,8 @@ def bench_batches(steps, warmup):
 def bench_conv(steps, warmup):
     """Data synthesis by convolution (fftconv_p / sum_fftconv path): 4096
     columns (64 x 64 A-scans) of 2000 samples convolved with one 300-sample
-    pulse, FP64 and FP32, as multiply-adds per second."""
+    pulse, FP64 and FP32, as multiply-adds (of the direct sum) per second;
+    and a long-filter case through both paths."""
     import torch
 
     from dream_amd.torch_ops import conv_sum
@@ -481,6 +482,23 @@ def bench_conv(steps, warmup):
         ms = time_launches(lambda: conv_sum(A, b, out=y), steps, warmup)
         out[f"conv_4096x2000_by_300_{name}"] = dict(ms=round(ms, 4),
                                                    macs_per_s=4096 * 2000 * 300 / (ms * 1e-3))
+    # Long filters (impulse responses convolved with each other, the
+    # reference's double-path synthesis): 1024 columns of 2000 samples by a
+    # 2000-sample filter, FFT path against the direct sums.
+    g = torch.Generator(device="cuda").manual_seed(8)
+    A = torch.randn((1024, 2000), generator=g, device="cuda", dtype=torch.float64)
+    b = torch.randn(2000, generator=g, device="cuda", dtype=torch.float64)
+    y = torch.empty((1024, 3999), device="cuda", dtype=torch.float64)
+    entry = {}
+    for path in ("fft", "direct"):
+        os.environ["DREAM_HIP_CONV"] = path
+        try:
+            ms = time_launches(lambda: conv_sum(A, b, out=y), max(2, steps // 10), 2)
+        finally:
+            os.environ.pop("DREAM_HIP_CONV", None)
+        entry[f"{path}_ms"] = round(ms, 4)
+    entry["macs_per_s_fft"] = 1024 * 2000 * 2000 / (entry["fft_ms"] * 1e-3)
+    out["conv_1024x2000_by_2000_f64"] = entry
     return out
 
 
