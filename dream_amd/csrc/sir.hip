@@ -19,6 +19,11 @@
 // run-length (bin, sum) in registers and touching LDS only when the bin
 // changes.  Rows are handed to lanes in a strided order so the 64 lanes of
 // a wave see rows spread over the whole element and hit different bins.
+// The LDS atomics of those flushes bound the kernel, and along x the bin
+// changes about every other step when the element lies to the side of the
+// point; for flat rectangles a point may instead walk columns (y inner, in
+// row segments), where |ry| <= b/2 keeps the bin for several steps.  Each
+// point takes the walk its run-length model prices lower.
 //
 // Sample index.  The index must equal the reference's FP64 sequence bit for
 // bit.  A fast estimate u = r*c1 - d/dt (r from the hardware rsqrt, about
@@ -38,6 +43,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "sir.hpp"
 #include "tune.hpp"
@@ -50,15 +56,20 @@ constexpr int kThreads = 256;
 constexpr double kBoundScale = 0x1p-36;
 
 // Sample index (as a double) of the sub-element at offset (rx, ry, rz), and
-// h = 1/(2 r) for its amplitude; ryy = ry*ry, rzz = rz*rz, ryz = ryy + rzz.
-// The estimate takes s = rx*rx + ryz in one FMA (within 2^-50 of the
-// reference's (rx*rx + ry*ry) + rz*rz, far inside the certification
-// margin), r and 1/(2r) from the hardware rsqrt seed and one Goldschmidt
-// step; the exact path recomputes s in the reference's order.  `lim` is
-// 0.5 - B (see the file comment): |u - rint(u)| <= lim certifies rint(u).
-__device__ __forceinline__ double sir_index(double rx, double ryy, double rzz, double ryz,
-                                            double c1, double c2, double lim, double cp, double d,
-                                            double dt, double &h)
+// h = 1/(2 r) for its amplitude.  sir_fast takes s = v*v + w in one FMA (v
+// the coordinate that varies along the walk, w the sum of the other two
+// squares; within 2^-50 of the reference's (rx*rx + ry*ry) + rz*rz, far
+// inside the certification margin), r and 1/(2r) from the hardware rsqrt
+// seed and one Goldschmidt step, and reports whether rint(u) is certified:
+// `lim` is 0.5 - B (see the file comment).  sir_exact recomputes the index
+// in the reference's order (ryy = ry*ry, rzz = rz*rz); a NaN index
+// (non-finite inputs) comes back as -1, out of range as the reference's
+// integer conversion of NaN is.  REFINE_H takes h through the Goldschmidt
+// step too (about 2^-47); without it h is the seed's 1/(2r), about 2^-24 --
+// for FP32 output, whose own rounding is of that size.
+template <bool REFINE_H = true>
+__device__ __forceinline__ bool sir_fast(double rx, double ryz, double c1, double c2, double lim,
+                                         double &ru, double &h)
 {
   const double s = __builtin_fma(rx, rx, ryz);
   const double y = __builtin_amdgcn_rsq(s);
@@ -66,16 +77,41 @@ __device__ __forceinline__ double sir_index(double rx, double ryy, double rzz, d
   h = 0.5 * y;
   const double e = __builtin_fma(-g, h, 0.5);
   g = __builtin_fma(g, e, g);
-  h = __builtin_fma(h, e, h);
+  if (REFINE_H) h = __builtin_fma(h, e, h);
   const double u = __builtin_fma(g, c1, -c2);
-  double ru = __builtin_rint(u);
-  if (!(__builtin_fabs(u - ru) <= lim)) {        // also s = 0, NaN, inf
-    const double re = __builtin_sqrt((rx * rx + ryy) + rzz);
-    const double t = (re * 1000.0) / cp;
-    ru = __builtin_rint((t - d) / dt);
-    h = 0.5 / re;
-  }
+  ru = __builtin_rint(u);
+  return __builtin_fabs(u - ru) <= lim;          // false also for s = 0, NaN, inf
+}
+
+// The reference's own sequence: correctly rounded sqrt and divisions.
+__device__ __forceinline__ void sir_exact(double rx, double ryy, double rzz, double cp, double d,
+                                          double dt, double &ru, double &h)
+{
+  const double re = __builtin_sqrt((rx * rx + ryy) + rzz);
+  const double t = (re * 1000.0) / cp;
+  ru = __builtin_rint((t - d) / dt);
+  if (ru != ru) ru = -1.0;
+  h = 0.5 / re;
+}
+
+template <bool REFINE_H = true>
+__device__ __forceinline__ double sir_index(double rx, double ryy, double rzz, double ryz,
+                                            double c1, double c2, double lim, double cp, double d,
+                                            double dt, double &h)
+{
+  double ru;
+  if (!sir_fast<REFINE_H>(rx, ryz, c1, c2, lim, ru, h)) sir_exact(rx, ryy, rzz, cp, d, dt, ru, h);
   return ru;
+}
+
+// v_cvt_i32_f64 saturates: values past the int32 range give INT_MIN or
+// INT_MAX (where the C conversion is undefined), so for an integral x that
+// is not NaN, x >= 0 && x < nt is one unsigned compare of the result.
+__device__ __forceinline__ int sat_i32(double x)
+{
+  int r;
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(x));
+  return r;
 }
 
 // Address-space views of the plan's tables: uniform reads of the
@@ -113,12 +149,60 @@ __device__ __forceinline__ int find_problem(const SirDev *P, int n, int64_t item
   return lo;
 }
 
-// One work item: observation point n of problem P, (element,row) chunk c.
-// The run-length sums and the LDS histogram are FP64 for both output
-// precisions: gfx950's LDS FP64 atomic add is about 20 times faster than its
-// FP32 one, and the column is rounded to T once, when it is written.
+// Double sum over the wave, in a fixed order (the same on every work-group).
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Whether point (xo, yo, zo) of a rectangle problem walks columns (y inner)
+// rather than rows (x inner).  A lane flushes its run into the histogram
+// when the bin changes, which along x happens at a rate of about
+// min(1, |rx|/r dx/l) per step (l = cp dt/1000, one sample of path) and
+// along y at min(1, |ry|/r dy/l); with each element's mean offsets for rx
+// and ry, the cheaper walk counts a flush as 4 contributions and charges
+// each walk its partly filled lane rounds.  Every work-group of the point
+// computes the same sums in the same order, so all its items agree.
+__device__ __forceinline__ bool walk_columns(const SirDev &P, double xo, double yo, double zo, double *red)
+{
+  gdouble *el = (gdouble *) P.el;
+  const int L = P.L;
+  double fx = 0.0, fy = 0.0;
+  for (int e = threadIdx.x; e < L; e += kThreads) {
+    const double mx = __builtin_fmax(__builtin_fabs(xo - el[e]), P.wx);
+    const double my = __builtin_fmax(__builtin_fabs(yo - el[L + e]), P.wy);
+    const double mz = zo - el[2 * L + e];
+    const double r = __builtin_sqrt(mx * mx + my * my + mz * mz);
+    if (r > 0.0) {
+      fx += __builtin_fmin(1.0, mx / r * P.sx);
+      fy += __builtin_fmin(1.0, my / r * P.sy);
+    }
+  }
+  fx = wave_sum_d(fx);
+  fy = wave_sum_d(fy);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { red[w] = fx; red[4 + w] = fy; }
+  __syncthreads();
+  fx = ((red[0] + red[1]) + red[2]) + red[3];
+  fy = ((red[4] + red[5]) + red[6]) + red[7];
+  fx /= L;
+  fy /= L;
+  const double nx = P.nx, S = P.seg_len;
+  const double rounds_r = __builtin_ceil((double) L * P.ny / P.chunks / kThreads);
+  const double rounds_c = __builtin_ceil((double) L * P.nx * P.nseg / P.chunks / kThreads);
+  const double cost_r = rounds_r * nx * (1.0 + 4.0 * (1.0 + (nx - 1.0) * fx) / nx);
+  const double cost_c = rounds_c * S * (1.0 + 4.0 * (1.0 + (S - 1.0) * fy) / S);
+  return __builtin_amdgcn_readfirstlane((int) (cost_c < cost_r)) != 0;
+}
+
+// One work item: observation point n of problem P, chunk c of its rows (or
+// of its column segments).  The run-length sums and the LDS histogram are
+// FP64 for both output precisions: gfx950's LDS FP64 atomic add is about 20
+// times faster than its FP32 one, and the column is rounded to T once, when
+// it is written.
 template <typename T, bool ANY_GLOBAL>
-__device__ void sir_item(const SirDev &P, int64_t local, double *hist)
+__device__ __forceinline__ void sir_item(const SirDev &P, int64_t local, double *hist, double *red)
 {
   const int64_t n = local / P.chunks;
   const int c = (int) (local - n * P.chunks);
@@ -126,14 +210,13 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
   const double xo = Ro[n], yo = Ro[n + P.No], zo = Ro[n + 2 * P.No];
   const double dlay = P.delay_multi ? ((gdouble *) P.delay)[n] : ((gdouble *) P.delay)[0];
   const int ny = P.ny, nx = P.nx, L = P.L;
-  const int Q = L * ny;
-  const int q0 = (int) ((int64_t) c * Q / P.chunks);
-  const int q1 = (int) ((int64_t) (c + 1) * Q / P.chunks);
   const int64_t nt = P.nt;
   const double ntd = (double) nt;
   T *out = reinterpret_cast<T *>(P.h) + n * nt;
   // Plans without a global-mode problem compile the LDS path only.
   const bool use_lds = !ANY_GLOBAL || P.mode != SIR_GLOBAL;
+  constexpr bool REFINE_H = !std::is_same<T, float>::value;
+  const bool columns = P.nseg > 0 && (P.walk == 1 || walk_columns(P, xo, yo, zo, red));
 
   if (use_lds) {
     for (int i = threadIdx.x; i < nt; i += kThreads) hist[i] = 0.0;
@@ -147,78 +230,166 @@ __device__ void sir_item(const SirDev &P, int64_t local, double *hist)
   // there); in global mode -1 marks "no run yet".
   int cur = use_lds ? (int) nt : -1;
   double acc = 0.0;
+  // ai = amp / r = (2 amp) h, added into the run by one FMA.
+  double amp2 = 0.0;
   gdouble *el = (gdouble *) P.el;
   gdouble *ys = (gdouble *) P.ys;
-  cdouble *xs = (cdouble *) P.xs;
-  // Rows q = e*ny + j are dealt to lanes with stride kThreads; (e, j, row)
-  // advance incrementally (row = j*row_stride mod ny, unchanged by j -= ny).
-  int q = q0 + (int) threadIdx.x;
-  int e = q / ny;
-  int j = q - e * ny;
-  int row = (int) (((int64_t) j * P.row_stride) % ny);
-  // One Goldschmidt step (in sir_index): gfx950's FP64 rsqrt seed is
-  // accurate to about 2^-24, which one step takes to about 2^-47 -- enough
-  // for the index certification and for FP64 amplitudes.
-  for (; q < q1; q += kThreads) {
-    const double xe = xo - el[e];
-    const double ye = yo - el[L + e];
-    const double ze = zo - el[2 * L + e];
-    const double d = (dlay - el[3 * L + e]) - el[4 * L + e];
-    const double ry = ye - ys[row];
-    const double ryy = ry * ry;
-    // Cylinders: the row's depth on the curved surface (ze - 0.0 == ze for
-    // flat elements).
-    const double rz = ze - (P.shape == SIR_CYLIND ? ((gdouble *) P.zrow)[row] : 0.0);
-    const double rzz = rz * rz;
-    const double c2 = d * P.inv_dt;
-    const double lim = 0.5 - kBoundScale * (ntd + 2.0 + __builtin_fabs(c2));
-    const double ryz = ryy + rzz;
-    // ai = amp / r = (2 amp) h, added into the run by one FMA.
-    const double amp2 = 2.0 * (el[5 * L + e] * P.amp);
-    const unsigned oob_row = oob;
-    auto contribution = [&](double x) {
-      double h;
-      const double ru = sir_index(xe - x, ryy, rzz, ryz, P.c1, c2, lim, P.cp, d, P.dt, h);
-      if (ru >= 0.0 && ru < ntd) {
-        const int bin = (int) ru;
-        if (bin != cur) {
-          if (use_lds) lds_add(hist + cur, acc);
-          else if (cur >= 0) global_add(out + cur, (T) acc);
-          cur = bin;
-          acc = 0.0;
-        }
-        acc = __builtin_fma(amp2, h, acc);
-      } else {
-        oob++;
+  // The run-length update of one index (or the out-of-range count).
+  auto accumulate = [&](int bin, double h) {
+    if ((unsigned) bin < (unsigned) nt) {
+      if (bin != cur) {
+        if (use_lds) lds_add(hist + cur, acc);
+        else if (cur >= 0) global_add(out + cur, (T) acc);
+        cur = bin;
+        acc = 0.0;
       }
-    };
-    if (P.shape == SIR_CIRC) {
-      // A disc row: the reference's accumulated walk from this row's start.
-      double x = ((gdouble *) P.row_x0)[row];
-      const int cnt = (int) ((gdouble *) P.row_n)[row];
-      for (int i = 0; i < cnt; i++) {
-        contribution(x);
-        x += P.dx;
-      }
+      acc = __builtin_fma(amp2, h, acc);
     } else {
-      // Sub-element x positions four at a time, so their scalar loads share
-      // one wait.
-      int i = 0;
-      for (; i + 4 <= nx; i += 4) {
-        const double x0 = xs[i], x1 = xs[i + 1], x2 = xs[i + 2], x3 = xs[i + 3];
-        contribution(x0);
-        contribution(x1);
-        contribution(x2);
-        contribution(x3);
-      }
-      for (; i < nx; i++) contribution(xs[i]);
+      oob++;
     }
-    oob_last |= (e == L - 1) && oob != oob_row;
-    e += P.step_e;
-    j += P.step_j;
-    if (j >= ny) { j -= ny; e++; }
-    row += P.step_row;
-    if (row >= ny) row -= ny;
+  };
+  // One Goldschmidt step (in sir_fast): gfx950's FP64 rsqrt seed is
+  // accurate to about 2^-24, which one step takes to about 2^-47 -- enough
+  // for the index certification and for FP64 amplitudes.  Sub-element
+  // positions are taken four at a time: the four fast estimates are
+  // independent chains the compiler interleaves, and one branch covers the
+  // (rare) exact path of any of them.
+  if (!columns) {
+    const int Q = L * ny;
+    const int q0 = (int) ((int64_t) c * Q / P.chunks);
+    const int q1 = (int) ((int64_t) (c + 1) * Q / P.chunks);
+    cdouble *xs = (cdouble *) P.xs;
+    // Rows q = e*ny + j are dealt to lanes with stride kThreads; (e, j, row)
+    // advance incrementally (row = j*row_stride mod ny, unchanged by j -= ny).
+    int q = q0 + (int) threadIdx.x;
+    int e = q / ny;
+    int j = q - e * ny;
+    int row = (int) (((int64_t) j * P.row_stride) % ny);
+    for (; q < q1; q += kThreads) {
+      const double xe = xo - el[e];
+      const double ye = yo - el[L + e];
+      const double ze = zo - el[2 * L + e];
+      const double d = (dlay - el[3 * L + e]) - el[4 * L + e];
+      const double ry = ye - ys[row];
+      const double ryy = ry * ry;
+      // Cylinders: the row's depth on the curved surface (ze - 0.0 == ze
+      // for flat elements).
+      const double rz = ze - (P.shape == SIR_CYLIND ? ((gdouble *) P.zrow)[row] : 0.0);
+      const double rzz = rz * rz;
+      const double c2 = d * P.inv_dt;
+      const double lim = 0.5 - kBoundScale * (ntd + 2.0 + __builtin_fabs(c2));
+      const double ryz = ryy + rzz;
+      amp2 = 2.0 * (el[5 * L + e] * P.amp);
+      const unsigned oob_row = oob;
+      if (P.shape == SIR_CIRC) {
+        // A disc row: the reference's accumulated walk from this row's start.
+        double x = ((gdouble *) P.row_x0)[row];
+        const int cnt = (int) ((gdouble *) P.row_n)[row];
+        for (int i = 0; i < cnt; i++) {
+          double h;
+          const double ru = sir_index<REFINE_H>(xe - x, ryy, rzz, ryz, P.c1, c2, lim, P.cp, d,
+                                                P.dt, h);
+          accumulate(sat_i32(ru), h);
+          x += P.dx;
+        }
+      } else {
+        int i = 0;
+        for (; i + 4 <= nx; i += 4) {
+          const double x0 = xs[i], x1 = xs[i + 1], x2 = xs[i + 2], x3 = xs[i + 3];
+          double r0, r1, r2, r3, h0, h1, h2, h3;
+          const bool k0 = sir_fast<REFINE_H>(xe - x0, ryz, P.c1, c2, lim, r0, h0);
+          const bool k1 = sir_fast<REFINE_H>(xe - x1, ryz, P.c1, c2, lim, r1, h1);
+          const bool k2 = sir_fast<REFINE_H>(xe - x2, ryz, P.c1, c2, lim, r2, h2);
+          const bool k3 = sir_fast<REFINE_H>(xe - x3, ryz, P.c1, c2, lim, r3, h3);
+          if (!(k0 & k1 & k2 & k3)) {
+            if (!k0) sir_exact(xe - x0, ryy, rzz, P.cp, d, P.dt, r0, h0);
+            if (!k1) sir_exact(xe - x1, ryy, rzz, P.cp, d, P.dt, r1, h1);
+            if (!k2) sir_exact(xe - x2, ryy, rzz, P.cp, d, P.dt, r2, h2);
+            if (!k3) sir_exact(xe - x3, ryy, rzz, P.cp, d, P.dt, r3, h3);
+          }
+          accumulate(sat_i32(r0), h0);
+          accumulate(sat_i32(r1), h1);
+          accumulate(sat_i32(r2), h2);
+          accumulate(sat_i32(r3), h3);
+        }
+        for (; i < nx; i++) {
+          double h;
+          const double ru = sir_index<REFINE_H>(xe - xs[i], ryy, rzz, ryz, P.c1, c2, lim, P.cp,
+                                                d, P.dt, h);
+          accumulate(sat_i32(ru), h);
+        }
+      }
+      oob_last |= (e == L - 1) && oob != oob_row;
+      e += P.step_e;
+      j += P.step_j;
+      if (j >= ny) { j -= ny; e++; }
+      row += P.step_row;
+      if (row >= ny) row -= ny;
+    }
+  } else {
+    // Column walk of a flat rectangle: unit u = sg*(L*nx) + i*L + e is
+    // segment sg (rows [sg*seg_len, +seg_len)) of x column i of element e;
+    // neighbouring lanes take neighbouring elements, so the lanes of a wave
+    // share a segment (their row loads hit one address) and see different
+    // bins.  The fast estimate takes s = ry*ry + (rx*rx + rz*rz), the exact
+    // path the reference's (rx*rx + ry*ry) + rz*rz.
+    const int QC = L * nx;
+    const int U = QC * P.nseg;
+    const int u0 = (int) ((int64_t) c * U / P.chunks);
+    const int u1 = (int) ((int64_t) (c + 1) * U / P.chunks);
+    gdouble *xs = (gdouble *) P.xs;
+    int u = u0 + (int) threadIdx.x;
+    int sg = u / QC;
+    int i = (u - sg * QC) / L;
+    int e = u - sg * QC - i * L;
+    for (; u < u1; u += kThreads) {
+      const double xe = xo - el[e];
+      const double ye = yo - el[L + e];
+      const double ze = zo - el[2 * L + e];
+      const double d = (dlay - el[3 * L + e]) - el[4 * L + e];
+      const double rx = xe - xs[i];
+      const double rzz = ze * ze;
+      const double rxz = rx * rx + rzz;
+      const double c2 = d * P.inv_dt;
+      const double lim = 0.5 - kBoundScale * (ntd + 2.0 + __builtin_fabs(c2));
+      amp2 = 2.0 * (el[5 * L + e] * P.amp);
+      const unsigned oob_unit = oob;
+      const int j0 = sg * P.seg_len;
+      const int j1 = min(j0 + P.seg_len, ny);
+      int j = j0;
+      for (; j + 4 <= j1; j += 4) {
+        const double y0 = ye - ys[j], y1 = ye - ys[j + 1], y2 = ye - ys[j + 2],
+                     y3 = ye - ys[j + 3];
+        double r0, r1, r2, r3, h0, h1, h2, h3;
+        const bool k0 = sir_fast<REFINE_H>(y0, rxz, P.c1, c2, lim, r0, h0);
+        const bool k1 = sir_fast<REFINE_H>(y1, rxz, P.c1, c2, lim, r1, h1);
+        const bool k2 = sir_fast<REFINE_H>(y2, rxz, P.c1, c2, lim, r2, h2);
+        const bool k3 = sir_fast<REFINE_H>(y3, rxz, P.c1, c2, lim, r3, h3);
+        if (!(k0 & k1 & k2 & k3)) {
+          if (!k0) sir_exact(rx, y0 * y0, rzz, P.cp, d, P.dt, r0, h0);
+          if (!k1) sir_exact(rx, y1 * y1, rzz, P.cp, d, P.dt, r1, h1);
+          if (!k2) sir_exact(rx, y2 * y2, rzz, P.cp, d, P.dt, r2, h2);
+          if (!k3) sir_exact(rx, y3 * y3, rzz, P.cp, d, P.dt, r3, h3);
+        }
+        accumulate(sat_i32(r0), h0);
+        accumulate(sat_i32(r1), h1);
+        accumulate(sat_i32(r2), h2);
+        accumulate(sat_i32(r3), h3);
+      }
+      for (; j < j1; j++) {
+        const double ry = ye - ys[j];
+        double r, h;
+        if (!sir_fast<REFINE_H>(ry, rxz, P.c1, c2, lim, r, h))
+          sir_exact(rx, ry * ry, rzz, P.cp, d, P.dt, r, h);
+        accumulate(sat_i32(r), h);
+      }
+      oob_last |= (e == L - 1) && oob != oob_unit;
+      e += P.cstep_e;
+      i += P.cstep_i;
+      if (e >= L) { e -= L; i++; }
+      if (i >= nx) { i -= nx; sg++; }
+      sg += P.cstep_sg;
+    }
   }
   if (use_lds) lds_add(hist + cur, acc);
   else if (cur >= 0) global_add(out + cur, (T) acc);
@@ -253,12 +424,13 @@ sir_kernel(const SirDev *__restrict__ probs, int n_probs, int64_t item_lo, int64
 {
   extern __shared__ __align__(16) unsigned char smem[];
   double *hist = reinterpret_cast<double *>(smem);
+  __shared__ double red[8];
   // This launch serves items [item_lo, item_hi) (a chunk of a long plan).
   if (counter == nullptr) {
     const int64_t item = item_lo + blockIdx.x;
     if (item >= item_hi) return;
     const int p = find_problem(probs, n_probs, item);
-    sir_item<T, ANY_GLOBAL>(probs[p], item - probs[p].item_begin, hist);
+    sir_item<T, ANY_GLOBAL>(probs[p], item - probs[p].item_begin, hist, red);
     return;
   }
   // Persistent: pull items from counter[0]; the last work-group to leave
@@ -271,7 +443,7 @@ sir_kernel(const SirDev *__restrict__ probs, int n_probs, int64_t item_lo, int64
     __syncthreads();
     if (item >= item_hi) break;
     const int p = find_problem(probs, n_probs, item);
-    sir_item<T, ANY_GLOBAL>(probs[p], item - probs[p].item_begin, hist);
+    sir_item<T, ANY_GLOBAL>(probs[p], item - probs[p].item_begin, hist, red);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -601,6 +773,18 @@ std::vector<double> element_table(const SirProblem &p)
 
 // ------------------------------------------------------------------ plan
 
+// DREAM_HIP_SIR_WALK = rows | columns forces the sub-element walk of
+// rectangle problems (for comparisons and tests): 1 rows, 2 columns, 0 the
+// per-point choice.
+int walk_override()
+{
+  const char *e = std::getenv("DREAM_HIP_SIR_WALK");
+  if (!e) return 0;
+  if (std::strcmp(e, "rows") == 0) return 1;
+  if (std::strcmp(e, "columns") == 0) return 2;
+  return 0;
+}
+
 SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int target_items)
   : out_dtype_(out_dtype)
 {
@@ -624,7 +808,7 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
   for (size_t i = 0; i < P; i++) {
     const SirProblem &p = problems[i];
     const int64_t L = (int64_t) (p.G.size() / 3);
-    require(p.No >= 0 && p.nt >= 1, "SIR: need No >= 0 and nt >= 1");
+    require(p.No >= 0 && p.nt >= 1 && p.nt < (1ll << 31), "SIR: need No >= 0 and 1 <= nt < 2^31");
     require(L >= 1, "SIR: need at least one element");
     require(p.dx > 0 && p.dy > 0 && p.dt > 0, "SIR: dx, dy and dt must be positive");
     require(p.No == 0 || (p.Ro && p.delay && p.h), "SIR: null Ro, delay or h");
@@ -688,6 +872,38 @@ SirPlan::SirPlan(const std::vector<SirProblem> &problems, int out_dtype, int tar
       chunks = std::min<int64_t>(chunks, std::max<int64_t>(Q / kThreads, 1));
     }
     d.chunks = (int32_t) std::max<int64_t>(chunks, 1);
+    if (p.shape == SIR_RECT && nx > 0 && ny > 0 && walk_override() != 1) {
+      d.walk = walk_override() == 2 ? 1 : 0;
+      // Column segments: the count that minimises lane rounds per item x
+      // rows per segment x the run-length cost of a segment (a flush per
+      // segment plus one per eighth step or so), over at most ny/8
+      // segments.
+      const int64_t QC = L * nx;
+      double best = 0.0;
+      int64_t best_n = 1;
+      for (int64_t ns = 1; ns <= std::max<int64_t>(ny / 8, 1); ns++) {
+        const int64_t S = (ny + ns - 1) / ns;
+        const int64_t nseg = (ny + S - 1) / S;
+        const double rounds = std::ceil((double) (QC * nseg) / (double) d.chunks / kThreads);
+        const double c = rounds * (double) S * (1.0 + 4.0 * (1.0 + (double) (S - 1) * 0.15) / (double) S);
+        if (ns == 1 || c < best) { best = c; best_n = nseg; }
+      }
+      const int64_t S = (ny + best_n - 1) / best_n;
+      require(QC * best_n < (1ll << 31) - kThreads, "SIR: element x column count too large");
+      d.nseg = (int32_t) ((ny + S - 1) / S);
+      d.seg_len = (int32_t) S;
+      d.cstep_sg = (int32_t) (kThreads / QC);
+      d.cstep_i = (int32_t) ((kThreads % QC) / L);
+      d.cstep_e = (int32_t) ((kThreads % QC) % L);
+      double sxm = 0.0, sym = 0.0;
+      for (double v : xs[i]) sxm += std::fabs(v);
+      for (double v : ys[i]) sym += std::fabs(v);
+      const double lam = p.cp * p.dt / 1000.0;
+      d.wx = sxm / (double) nx;
+      d.wy = sym / (double) ny;
+      d.sx = p.dx / lam;
+      d.sy = p.dy / lam;
+    }
     const bool fits = (size_t) (p.nt + 1) * esz <= lds_cap;     // + the spare bin
     d.mode = !fits ? SIR_GLOBAL : (d.chunks > 1 ? SIR_LDS_ATOMIC : SIR_LDS_STORE);
     if (fits) max_nt_lds = std::max(max_nt_lds, p.nt);

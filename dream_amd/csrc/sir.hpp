@@ -45,7 +45,15 @@ struct SirDev {
   Status *status;
   int64_t item_begin;               // first work item of this problem
   int32_t mode;                     // SIR_* below
-  int32_t pad;
+  // Column walk (rectangles): the (element, x column) pairs, each cut into
+  // nseg segments of seg_len rows, as units u = sg*(L*nx) + i*L + e; u +=
+  // 256 steps (e, i, sg) by cstep_*.  0 segments: rows only; walk 1: every
+  // point walks columns (else each point picks its cheaper walk).
+  int32_t nseg;
+  int32_t seg_len, cstep_e, cstep_i, cstep_sg, walk;
+  // The walk choice: mean |xs| and |ys|, and dx, dy in samples of path
+  // (dx / (cp dt/1000), dy / (cp dt/1000)).
+  double wx, wy, sx, sy;
 };
 
 enum { SIR_LDS_STORE = 0, SIR_LDS_ATOMIC = 1, SIR_GLOBAL = 2 };

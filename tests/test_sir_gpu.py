@@ -174,6 +174,50 @@ def test_batch_mixing_global_and_lds_problems(gpu, oracle):
                 assert rel_l2(Hn, ref) <= tol
 
 
+@pytest.mark.parametrize("walk", ["rows", "columns", "auto"])
+def test_row_and_column_walks(gpu, oracle, monkeypatch, walk):
+    """Both sub-element walks of rectangles (rows: x inner; columns: y inner,
+    in row segments) and the per-point choice, forced by DREAM_HIP_SIR_WALK,
+    against the oracle: many points (column store), few points (items split
+    over work-groups, atomic column adds), a long SIR (global accumulation),
+    out-of-range counts and the last-element status rule, FP32 output."""
+    from dream_amd.torch_ops import arr_rect
+    if walk != "auto":
+        monkeypatch.setenv("DREAM_HIP_SIR_WALK", walk)
+    rng = np.random.default_rng(77)
+    L = 7
+    G = np.c_[np.linspace(-3, 3, L), rng.uniform(-1, 1, L), rng.uniform(-0.2, 0.2, L)]
+    cases = [(300, 0.4, 7.0, 0.02, 1500, "ignore"), (3, 1.1, 4.0, 0.025, 900, "ignore"),
+             (4, 0.5, 3.0, 0.005, 11000, "ignore"), (50, 0.7, 5.0, 0.02, 700, "stop")]
+    total_oob = 0
+    for No, a, b, dt, nt, lvl in cases:
+        Ro = np.c_[rng.uniform(-6, 6, No), rng.uniform(-4, 4, No), rng.uniform(6, 30, No)]
+        # Delays that put part of the arrivals before sample 0 and past nt.
+        delay = np.linalg.norm(Ro, axis=1) * 1000 / 1500 - rng.uniform(-1.0, 0.5 * nt * dt, No)
+        focal = rng.uniform(0, 0.5, L)
+        # The oracle under `stop` ends a thread at its first out-of-range
+        # contribution, so its sum and count come from `ignore`; the device
+        # computes everything at every level.
+        ref, _, oob = oracle.dream_arr_rect(Ro, a, b, 0.05, 0.05, dt, nt, delay, 1.0, 1500.0, G,
+                                            foc_met="ud", focal=focal, apod_met="hann",
+                                            err_level="ignore", nthreads=4)
+        _, err, _ = oracle.dream_arr_rect(Ro, a, b, 0.05, 0.05, dt, nt, delay, 1.0, 1500.0, G,
+                                          foc_met="ud", focal=focal, apod_met="hann",
+                                          err_level=lvl, nthreads=4)
+        Rt = torch.tensor(Ro.T.copy(), device="cuda")
+        dt_ = torch.tensor(delay, device="cuda")
+        H, code, n = arr_rect(Rt, a, b, 0.05, 0.05, dt, nt, dt_, 1.0, 1500.0, G, foc_met="ud",
+                              focal=focal, apod_met="hann", err_level=lvl)
+        assert n == oob and code == err, (No, n, oob, code, err)
+        total_oob += oob
+        assert np.max(np.abs(H.cpu().numpy().T - ref)) <= 1e-12 * np.abs(ref).max(), No
+        H32, _, n32 = arr_rect(Rt, a, b, 0.05, 0.05, dt, nt, dt_, 1.0, 1500.0, G, foc_met="ud",
+                               focal=focal, apod_met="hann", err_level="ignore",
+                               out_dtype=torch.float32)
+        assert n32 == oob and rel_l2(H32.double().cpu().numpy().T, ref) <= 2e-5
+    assert total_oob > 0
+
+
 def test_error_codes_gpu(gpu):
     from dream_amd import dream_arr_rect, dreamrect
     Ro = np.array([[0.0, 0.0, 20.0], [0.0, 0.0, 21.0], [0.0, 0.0, 22.0]])
