@@ -75,22 +75,22 @@ def _counter_value(path, key):
 
 
 def valu_issue_frac(contributions, ms):
-    """C3: VALU lane-instructions per second (the committed counters' lane
-    instructions per contribution x contributions / time) over the device's
-    rate (256 CUs x 64 lanes x 2.4 GHz), or None."""
-    per = _counter_value(C3_COUNTERS, "valu_lane_insts_per_contribution")
+    """C3: VALU busy lane-cycles per second (the committed counters' busy
+    lane-cycles per contribution -- an FP64 rsqrt holds its SIMD for four
+    passes -- x contributions / time) over the device's rate (256 CUs x 64
+    lanes x 2.4 GHz), or None."""
+    per = _counter_value(C3_COUNTERS, "valu_busy_lane_cycles_per_contribution")
     if per is None:
         return None
     return per * contributions / (ms * 1e-3) / (N_CU * 64 * CLOCK_HZ)
 
 
-def l1_lookup_frac(pairs, ms):
-    """C5: L1 tag lookups per second (the committed counters' lookups per
-    64-lane gather x gathers / time) over one lookup per CU per cycle, or None."""
-    per = _counter_value(C5_COUNTERS, "tag_lookups_per_gather")
-    if per is None:
-        return None
-    return per * (pairs / 64) / (ms * 1e-3) / (N_CU * CLOCK_HZ)
+def c5_bound():
+    """C5: the address/tag path of the gathers, as the committed counters
+    measured it (TA busy fraction and L1 tag lookups per gather, under the
+    profiler), or Nones."""
+    return dict(ta_busy_frac_profiled=_counter_value(C5_COUNTERS, "ta_busy_frac_profiled"),
+                tag_lookups_per_gather=_counter_value(C5_COUNTERS, "tag_lookups_per_gather"))
 
 
 def log(msg):
@@ -562,12 +562,11 @@ def bench_das_diff_c5(steps, warmup):
     alg_bytes = n_inb * 4 + F * plan.No * 4
     # Y (268 MB per frame) is re-read from L2/MALL by every voxel tile, so the
     # no-reuse byte count exceeds HBM bandwidth: it is not a bound for C5.
-    # The forward is bound by the L1 tag-lookup rate (one per CU cycle).
+    # The forward is bound by the gathers' address and tag path (TA busy).
     out["c5_rca_8frames_f32"] = dict(fwd_ms=round(fwd, 3), bwd_apod_ms=round(bwd, 3),
                                      bwd_over_fwd=bwd / fwd, sums_per_s=pairs / (fwd * 1e-3),
-                                     in_bounds_pairs=n_inb, bound="L1 tag lookups",
-                                     no_reuse_bytes_per_s=alg_bytes / (fwd * 1e-3),
-                                     l1_lookup_frac=l1_lookup_frac(pairs, fwd))
+                                     in_bounds_pairs=n_inb, bound="TA / L1 tag lookups",
+                                     no_reuse_bytes_per_s=alg_bytes / (fwd * 1e-3), **c5_bound())
     return out
 
 
