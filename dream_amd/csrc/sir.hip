@@ -53,54 +53,61 @@ namespace dream {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+// Sub-elements per block of independent index computations.
+constexpr int kBlock = 4;
 constexpr double kBoundScale = 0x1p-36;
 
 // Sample index (as a double) of the sub-element at offset (rx, ry, rz), and
-// h = 1/(2 r) for its amplitude.  sir_fast takes s = v*v + w in one FMA (v
-// the coordinate that varies along the walk, w the sum of the other two
-// squares; within 2^-50 of the reference's (rx*rx + ry*ry) + rz*rz, far
-// inside the certification margin), r and 1/(2r) from the hardware rsqrt
-// seed and one Goldschmidt step, and reports whether rint(u) is certified:
-// `lim` is 0.5 - B (see the file comment).  sir_exact recomputes the index
-// in the reference's order (ryy = ry*ry, rzz = rz*rz); a NaN index
-// (non-finite inputs) comes back as -1, out of range as the reference's
-// integer conversion of NaN is.  REFINE_H takes h through the Goldschmidt
-// step too (about 2^-47); without it h is the seed's 1/(2r), about 2^-24 --
-// for FP32 output, whose own rounding is of that size.
-template <bool REFINE_H = true>
-__device__ __forceinline__ bool sir_fast(double rx, double ryz, double c1, double c2, double lim,
-                                         double &ru, double &h)
+// q, proportional to 1/r, for its amplitude.  sir_fast takes s = v*v + w in
+// one FMA (v the coordinate that varies along the walk, w the sum of the
+// other two squares; within 2^-50 of the reference's (rx*rx + ry*ry) +
+// rz*rz, far inside the certification margin), the hardware rsqrt seed y
+// and one Newton step written as sqrt(s) = g (3 - g y) / 2 with g = s y --
+// the Goldschmidt pair's value g (1 + e), e = (1 - g y) / 2, relative error
+// 1.5 d^2 for a seed error d -- so that u = ((c1/2) g)(3 - g y) - c2 costs
+// four FP64 operations after the seed; it reports whether rint(u) is
+// certified (`lim` is 0.5 - B, see the file comment).  With REFINE, q = y (3
+// - g y) = 2/r to about 2^-47 (FP64 output), else q = y = 1/r to the seed's
+// 2^-24 (FP32 output, whose own rounding is of that size); the amplitude
+// weight carries the matching 1/2 or 1.  sir_exact recomputes the index in
+// the reference's order (ryy = ry*ry, rzz = rz*rz), with q = 2/re or 1/re;
+// a NaN index (non-finite inputs) comes back as -1, out of range as the
+// reference's integer conversion of NaN is.
+template <bool REFINE>
+__device__ __forceinline__ bool sir_fast(double v, double w, double c1h, double c2, double lim,
+                                         double &ru, double &q)
 {
-  const double s = __builtin_fma(rx, rx, ryz);
+  const double s = __builtin_fma(v, v, w);
   const double y = __builtin_amdgcn_rsq(s);
-  double g = s * y;
-  h = 0.5 * y;
-  const double e = __builtin_fma(-g, h, 0.5);
-  g = __builtin_fma(g, e, g);
-  if (REFINE_H) h = __builtin_fma(h, e, h);
-  const double u = __builtin_fma(g, c1, -c2);
+  const double g = s * y;
+  const double t = __builtin_fma(-g, y, 3.0);
+  const double u = __builtin_fma(c1h * g, t, -c2);
   ru = __builtin_rint(u);
+  q = REFINE ? y * t : y;
   return __builtin_fabs(u - ru) <= lim;          // false also for s = 0, NaN, inf
 }
 
 // The reference's own sequence: correctly rounded sqrt and divisions.
+template <bool REFINE>
 __device__ __forceinline__ void sir_exact(double rx, double ryy, double rzz, double cp, double d,
-                                          double dt, double &ru, double &h)
+                                          double dt, double &ru, double &q)
 {
   const double re = __builtin_sqrt((rx * rx + ryy) + rzz);
   const double t = (re * 1000.0) / cp;
   ru = __builtin_rint((t - d) / dt);
   if (ru != ru) ru = -1.0;
-  h = 0.5 / re;
+  q = (REFINE ? 2.0 : 1.0) / re;
 }
 
-template <bool REFINE_H = true>
+template <bool REFINE>
 __device__ __forceinline__ double sir_index(double rx, double ryy, double rzz, double ryz,
-                                            double c1, double c2, double lim, double cp, double d,
-                                            double dt, double &h)
+                                            double c1h, double c2, double lim, double cp,
+                                            double d, double dt, double &q)
 {
   double ru;
-  if (!sir_fast<REFINE_H>(rx, ryz, c1, c2, lim, ru, h)) sir_exact(rx, ryy, rzz, cp, d, dt, ru, h);
+  if (!sir_fast<REFINE>(rx, ryz, c1h, c2, lim, ru, q))
+    sir_exact<REFINE>(rx, ryy, rzz, cp, d, dt, ru, q);
   return ru;
 }
 
@@ -182,10 +189,14 @@ __device__ __forceinline__ bool walk_columns(const SirDev &P, double xo, double 
   fx = wave_sum_d(fx);
   fy = wave_sum_d(fy);
   const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) { red[w] = fx; red[4 + w] = fy; }
+  if ((threadIdx.x & 63) == 0) { red[w] = fx; red[kWaves + w] = fy; }
   __syncthreads();
-  fx = ((red[0] + red[1]) + red[2]) + red[3];
-  fy = ((red[4] + red[5]) + red[6]) + red[7];
+  fx = red[0];
+  fy = red[kWaves];
+  for (int k = 1; k < kWaves; k++) {
+    fx += red[k];
+    fy += red[kWaves + k];
+  }
   fx /= L;
   fy /= L;
   const double nx = P.nx, S = P.seg_len;
@@ -230,30 +241,62 @@ __device__ __forceinline__ void sir_item(const SirDev &P, int64_t local, double 
   // there); in global mode -1 marks "no run yet".
   int cur = use_lds ? (int) nt : -1;
   double acc = 0.0;
-  // ai = amp / r = (2 amp) h, added into the run by one FMA.
-  double amp2 = 0.0;
+  // ai = amp / r = (amp / 2) q (or amp q without REFINE), added into the
+  // run by one FMA; ampq is that weight.
+  double ampq = 0.0;
+  const double c1h = 0.5 * P.c1;
   gdouble *el = (gdouble *) P.el;
   gdouble *ys = (gdouble *) P.ys;
-  // The run-length update of one index (or the out-of-range count).
-  auto accumulate = [&](int bin, double h) {
-    if ((unsigned) bin < (unsigned) nt) {
-      if (bin != cur) {
-        if (use_lds) lds_add(hist + cur, acc);
-        else if (cur >= 0) global_add(out + cur, (T) acc);
-        cur = bin;
-        acc = 0.0;
-      }
-      acc = __builtin_fma(amp2, h, acc);
-    } else {
-      oob++;
+  // The run-length update of one in-range index.
+  auto run = [&](int bin, double q) {
+    if (bin != cur) {
+      if (use_lds) lds_add(hist + cur, acc);
+      else if (cur >= 0) global_add(out + cur, (T) acc);
+      cur = bin;
+      acc = 0.0;
     }
+    acc = __builtin_fma(ampq, q, acc);
   };
-  // One Goldschmidt step (in sir_fast): gfx950's FP64 rsqrt seed is
-  // accurate to about 2^-24, which one step takes to about 2^-47 -- enough
-  // for the index certification and for FP64 amplitudes.  Sub-element
-  // positions are taken four at a time: the four fast estimates are
-  // independent chains the compiler interleaves, and one branch covers the
-  // (rare) exact path of any of them.
+  // One index: the run-length update, or the per-lane out-of-range count.
+  auto accumulate = [&](int bin, double q) {
+    if ((unsigned) bin < (unsigned) nt) run(bin, q);
+    else oob++;
+  };
+  // kBlock indices in a loop whose trip count is the same on every active
+  // lane: the out-of-range ones are counted for the whole wave from the
+  // compare masks (scalar bit counts into cnt_w) and marked per lane in
+  // out_mask, so counting costs no vector instruction per index.
+  auto accumulate_block = [&](const double (&r)[kBlock], const double (&q)[kBlock],
+                              unsigned &cnt_w, unsigned long long &out_mask) {
+    int bin[kBlock];
+    bool in[kBlock];
+    const unsigned long long act = __ballot(1);
+    unsigned long long all_in = act;
+    int n_in = 0;
+#pragma unroll
+    for (int t = 0; t < kBlock; t++) {
+      bin[t] = sat_i32(r[t]);
+      in[t] = (unsigned) bin[t] < (unsigned) nt;
+      const unsigned long long m = __ballot(in[t]);
+      n_in += __popcll(m);
+      all_in &= m;
+    }
+    cnt_w += (unsigned) (kBlock * __popcll(act) - n_in);
+    out_mask |= act & ~all_in;
+#pragma unroll
+    for (int t = 0; t < kBlock; t++)
+      if (in[t]) run(bin[t], q[t]);
+  };
+  // After such a loop: the wave's count goes to its lowest active lane (in
+  // oob_s, apart from the per-lane counts in oob), and whether this lane saw
+  // one comes back.
+  unsigned oob_s = 0;
+  auto settle = [&](unsigned cnt_w, unsigned long long out_mask) {
+    const unsigned long long act = __ballot(1);
+    const int lane = (int) (threadIdx.x & 63);
+    if (lane == __ffsll((long long) act) - 1) oob_s += cnt_w;
+    return ((out_mask >> lane) & 1ull) != 0;
+  };
   if (!columns) {
     const int Q = L * ny;
     const int q0 = (int) ((int64_t) c * Q / P.chunks);
@@ -279,47 +322,48 @@ __device__ __forceinline__ void sir_item(const SirDev &P, int64_t local, double 
       const double c2 = d * P.inv_dt;
       const double lim = 0.5 - kBoundScale * (ntd + 2.0 + __builtin_fabs(c2));
       const double ryz = ryy + rzz;
-      amp2 = 2.0 * (el[5 * L + e] * P.amp);
+      ampq = (el[5 * L + e] * P.amp) * (REFINE_H ? 0.5 : 1.0);
       const unsigned oob_row = oob;
+      bool row_out = false;
+      unsigned cnt_w = 0;
+      unsigned long long out_mask = 0;
       if (P.shape == SIR_CIRC) {
         // A disc row: the reference's accumulated walk from this row's start.
         double x = ((gdouble *) P.row_x0)[row];
         const int cnt = (int) ((gdouble *) P.row_n)[row];
         for (int i = 0; i < cnt; i++) {
           double h;
-          const double ru = sir_index<REFINE_H>(xe - x, ryy, rzz, ryz, P.c1, c2, lim, P.cp, d,
+          const double ru = sir_index<REFINE_H>(xe - x, ryy, rzz, ryz, c1h, c2, lim, P.cp, d,
                                                 P.dt, h);
           accumulate(sat_i32(ru), h);
           x += P.dx;
         }
       } else {
         int i = 0;
-        for (; i + 4 <= nx; i += 4) {
-          const double x0 = xs[i], x1 = xs[i + 1], x2 = xs[i + 2], x3 = xs[i + 3];
-          double r0, r1, r2, r3, h0, h1, h2, h3;
-          const bool k0 = sir_fast<REFINE_H>(xe - x0, ryz, P.c1, c2, lim, r0, h0);
-          const bool k1 = sir_fast<REFINE_H>(xe - x1, ryz, P.c1, c2, lim, r1, h1);
-          const bool k2 = sir_fast<REFINE_H>(xe - x2, ryz, P.c1, c2, lim, r2, h2);
-          const bool k3 = sir_fast<REFINE_H>(xe - x3, ryz, P.c1, c2, lim, r3, h3);
-          if (!(k0 & k1 & k2 & k3)) {
-            if (!k0) sir_exact(xe - x0, ryy, rzz, P.cp, d, P.dt, r0, h0);
-            if (!k1) sir_exact(xe - x1, ryy, rzz, P.cp, d, P.dt, r1, h1);
-            if (!k2) sir_exact(xe - x2, ryy, rzz, P.cp, d, P.dt, r2, h2);
-            if (!k3) sir_exact(xe - x3, ryy, rzz, P.cp, d, P.dt, r3, h3);
+        for (; i + kBlock <= nx; i += kBlock) {
+          double r[kBlock], w[kBlock];
+          bool k[kBlock], all = true;
+#pragma unroll
+          for (int t = 0; t < kBlock; t++) {
+            k[t] = sir_fast<REFINE_H>(xe - xs[i + t], ryz, c1h, c2, lim, r[t], w[t]);
+            all &= k[t];
           }
-          accumulate(sat_i32(r0), h0);
-          accumulate(sat_i32(r1), h1);
-          accumulate(sat_i32(r2), h2);
-          accumulate(sat_i32(r3), h3);
+          if (!all) {
+#pragma unroll
+            for (int t = 0; t < kBlock; t++)
+              if (!k[t]) sir_exact<REFINE_H>(xe - xs[i + t], ryy, rzz, P.cp, d, P.dt, r[t], w[t]);
+          }
+          accumulate_block(r, w, cnt_w, out_mask);
         }
+        row_out = settle(cnt_w, out_mask);
         for (; i < nx; i++) {
           double h;
-          const double ru = sir_index<REFINE_H>(xe - xs[i], ryy, rzz, ryz, P.c1, c2, lim, P.cp,
+          const double ru = sir_index<REFINE_H>(xe - xs[i], ryy, rzz, ryz, c1h, c2, lim, P.cp,
                                                 d, P.dt, h);
           accumulate(sat_i32(ru), h);
         }
       }
-      oob_last |= (e == L - 1) && oob != oob_row;
+      oob_last |= (e == L - 1) && (oob != oob_row || row_out);
       e += P.step_e;
       j += P.step_j;
       if (j >= ny) { j -= ny; e++; }
@@ -352,38 +396,44 @@ __device__ __forceinline__ void sir_item(const SirDev &P, int64_t local, double 
       const double rxz = rx * rx + rzz;
       const double c2 = d * P.inv_dt;
       const double lim = 0.5 - kBoundScale * (ntd + 2.0 + __builtin_fabs(c2));
-      amp2 = 2.0 * (el[5 * L + e] * P.amp);
+      ampq = (el[5 * L + e] * P.amp) * (REFINE_H ? 0.5 : 1.0);
       const unsigned oob_unit = oob;
+      unsigned cnt_w = 0;
+      unsigned long long out_mask = 0;
       const int j0 = sg * P.seg_len;
       const int j1 = min(j0 + P.seg_len, ny);
+      // Every segment has seg_len rows but the last, so the wave's whole
+      // blocks of kBlock rows are a uniform count: the shorter length's when
+      // any of its lanes holds a last segment.
+      const int last_len = ny - (P.nseg - 1) * P.seg_len;
+      const int nb = (__ballot(j1 - j0 < P.seg_len) ? last_len : P.seg_len) / kBlock;
       int j = j0;
-      for (; j + 4 <= j1; j += 4) {
-        const double y0 = ye - ys[j], y1 = ye - ys[j + 1], y2 = ye - ys[j + 2],
-                     y3 = ye - ys[j + 3];
-        double r0, r1, r2, r3, h0, h1, h2, h3;
-        const bool k0 = sir_fast<REFINE_H>(y0, rxz, P.c1, c2, lim, r0, h0);
-        const bool k1 = sir_fast<REFINE_H>(y1, rxz, P.c1, c2, lim, r1, h1);
-        const bool k2 = sir_fast<REFINE_H>(y2, rxz, P.c1, c2, lim, r2, h2);
-        const bool k3 = sir_fast<REFINE_H>(y3, rxz, P.c1, c2, lim, r3, h3);
-        if (!(k0 & k1 & k2 & k3)) {
-          if (!k0) sir_exact(rx, y0 * y0, rzz, P.cp, d, P.dt, r0, h0);
-          if (!k1) sir_exact(rx, y1 * y1, rzz, P.cp, d, P.dt, r1, h1);
-          if (!k2) sir_exact(rx, y2 * y2, rzz, P.cp, d, P.dt, r2, h2);
-          if (!k3) sir_exact(rx, y3 * y3, rzz, P.cp, d, P.dt, r3, h3);
+      for (int blk = 0; blk < nb; blk++, j += kBlock) {
+        double yv[kBlock], r[kBlock], w[kBlock];
+        bool k[kBlock], all = true;
+#pragma unroll
+        for (int t = 0; t < kBlock; t++) yv[t] = ye - ys[j + t];
+#pragma unroll
+        for (int t = 0; t < kBlock; t++) {
+          k[t] = sir_fast<REFINE_H>(yv[t], rxz, c1h, c2, lim, r[t], w[t]);
+          all &= k[t];
         }
-        accumulate(sat_i32(r0), h0);
-        accumulate(sat_i32(r1), h1);
-        accumulate(sat_i32(r2), h2);
-        accumulate(sat_i32(r3), h3);
+        if (!all) {
+#pragma unroll
+          for (int t = 0; t < kBlock; t++)
+            if (!k[t]) sir_exact<REFINE_H>(rx, yv[t] * yv[t], rzz, P.cp, d, P.dt, r[t], w[t]);
+        }
+        accumulate_block(r, w, cnt_w, out_mask);
       }
+      const bool unit_out = settle(cnt_w, out_mask);
       for (; j < j1; j++) {
         const double ry = ye - ys[j];
         double r, h;
-        if (!sir_fast<REFINE_H>(ry, rxz, P.c1, c2, lim, r, h))
-          sir_exact(rx, ry * ry, rzz, P.cp, d, P.dt, r, h);
+        if (!sir_fast<REFINE_H>(ry, rxz, c1h, c2, lim, r, h))
+          sir_exact<REFINE_H>(rx, ry * ry, rzz, P.cp, d, P.dt, r, h);
         accumulate(sat_i32(r), h);
       }
-      oob_last |= (e == L - 1) && oob != oob_unit;
+      oob_last |= (e == L - 1) && (oob != oob_unit || unit_out);
       e += P.cstep_e;
       i += P.cstep_i;
       if (e >= L) { e -= L; i++; }
@@ -395,7 +445,7 @@ __device__ __forceinline__ void sir_item(const SirDev &P, int64_t local, double 
   else if (cur >= 0) global_add(out + cur, (T) acc);
 
   // Status: wave-reduced, one atomic per wave.
-  const unsigned wsum = wave_sum(oob);
+  const unsigned wsum = wave_sum(oob + oob_s);
   const bool any_last = __any(oob_last);
   if ((threadIdx.x & 63) == 0 && wsum) {
     atomicOr(&P.status->any_oob, 1u);
@@ -417,14 +467,18 @@ __device__ __forceinline__ void sir_item(const SirDev &P, int64_t local, double 
   }
 }
 
+// Five waves per SIMD: the LDS histogram of a 4000-sample SIR already limits
+// a CU to five 256-thread work-groups, and holding the registers to that
+// occupancy (a few spilled) beat fewer, register-rich waves and larger
+// work-groups on the uneven batch.
 template <typename T, bool ANY_GLOBAL>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5)))
 sir_kernel(const SirDev *__restrict__ probs, int n_probs, int64_t item_lo, int64_t item_hi,
            unsigned long long *counter)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   double *hist = reinterpret_cast<double *>(smem);
-  __shared__ double red[8];
+  __shared__ double red[2 * kWaves];
   // This launch serves items [item_lo, item_hi) (a chunk of a long plan).
   if (counter == nullptr) {
     const int64_t item = item_lo + blockIdx.x;
@@ -472,13 +526,13 @@ __global__ void rect_indices_kernel(double xo, double yo, double zo, const doubl
   sqrt_rsqrt<1>((rx * rx + ry * ry) + zo * zo, r, rinv);
   const double umax = __builtin_fabs(__builtin_fma(r, c1, -c2));
   const double lim = 0.5 - kBoundScale * (umax + 2.0 + __builtin_fabs(c2));
-  double h;
-  it_out[q] = (int64_t) sir_index(rx, ry * ry, zo * zo, ry * ry + zo * zo, c1, c2, lim, cp, delay,
-                                  dt, h);
+  double w;
+  it_out[q] = (int64_t) sir_index<true>(rx, ry * ry, zo * zo, ry * ry + zo * zo, 0.5 * c1, c2, lim,
+                                        cp, delay, dt, w);
 }
 
-// The hardware FP64 rsqrt seed of each s (the first step of sir_estimate
-// and saft_pos), for tests of the accuracy the certification assumes.
+// The hardware FP64 rsqrt seed of each s (the first step of sir_fast and
+// saft_pos), for tests of the accuracy the certification assumes.
 __global__ void rsq_seed_kernel(const double *s, int64_t n, double *y)
 {
   const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;

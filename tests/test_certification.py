@@ -3,11 +3,12 @@ by emulating their arithmetic on the CPU (correctly rounded FMAs through
 exact rationals) with an rsqrt seed at the worst relative error measured on
 gfx950 (2^-24.2) and beyond it (2^-23).
 
-SIR (sir.hip, `sir_estimate`): s = fma(rx, rx, ry^2 + rz^2), one
-Goldschmidt step, u = fma(r, c1, -c2); rint(u) is kept when
-|u - rint(u)| <= 1/2 - 2^-36 (nt + 2 + |c2|).  The exact index is the
-reference's rint(((sqrt((rx*rx + ry*ry) + rz*rz) * 1000) / cp - d) / dt)
-(src/dreamrect.cc:306-314).
+SIR (sir.hip, `sir_fast`): s = fma(v, v, w) (v the walked coordinate, w
+the other two squares summed), g = s y, t = fma(-g, y, 3), u = fma((c1/2) g,
+t, -c2) -- one Newton step folded into the index -- and the amplitude
+factor q = y t (2/r); rint(u) is kept when |u - rint(u)| <= 1/2 - 2^-36 (nt
++ 2 + |c2|).  The exact index is the reference's rint(((sqrt((rx*rx +
+ry*ry) + rz*rz) * 1000) / cp - d) / dt) (src/dreamrect.cc:306-314).
 
 SAFT (das.hip, `saft_pos`): u = (r / cp + delay) (2 fs) with the fast r; it
 is kept when it lies farther than 2^-29 |u| + 2^-40 (|2 fs delay| + 1) from
@@ -38,6 +39,31 @@ def fast_root(s, e0):
     h = 0.5 * y
     e = fma(-g, h, 0.5)
     return fma(g, e, g), fma(h, e, h)
+
+
+def sir_fast_u(v, w, c1, c2, e0):
+    """The SIR kernel's u and q from a seed with relative error e0."""
+    s = fma(v, v, w)
+    y = (1.0 / math.sqrt(s)) * (1.0 + e0)
+    g = s * y
+    t = fma(-g, y, 3.0)
+    return fma((0.5 * c1) * g, t, -c2), y * t
+
+
+def test_sir_newton_form_error():
+    """u = (c1/2) g (3 - g y) - c2 from a 2^-24.2 seed is within 2^-46 of
+    c1 sqrt(s) - c2 (relative to |u| + |c2| + 1), and q = y (3 - g y) within
+    2^-46 of 2/sqrt(s)."""
+    rng = np.random.default_rng(4)
+    for _ in range(2000):
+        v, w = rng.uniform(-20, 20), rng.uniform(1e-2, 1e4)
+        c1, c2 = 1000.0 / (1500.0 * rng.choice([0.02, 0.025])), rng.uniform(-3000, 3000)
+        for e0 in SEED_ERRORS[:2]:
+            u, q = sir_fast_u(v, w, c1, c2, e0)
+            r = math.sqrt(Fraction(v) * Fraction(v) + Fraction(w))
+            u_true = c1 * r - c2
+            assert abs(u - u_true) <= 2.0 ** -46 * (abs(u_true) + abs(c2) + 1.0)
+            assert abs(q * r / 2.0 - 1.0) <= 2.0 ** -46
 
 
 def test_goldschmidt_step_error():
@@ -76,8 +102,10 @@ def test_sir_certified_index_equals_exact():
         c1, c2 = 1000.0 / (cp * dt), d * (1.0 / dt)
         lim = 0.5 - 2.0 ** -36 * (nt + 2.0 + abs(c2))
         for e0 in SEED_ERRORS:
-            g, _ = fast_root(fma(rx, rx, ryy + rzz), e0)
-            u = fma(g, c1, -c2)
+            # Row walk (x varies: w = ry^2 + rz^2) and column walk (y varies:
+            # w = rx^2 + rz^2) take different roundings of s.
+            w_, v_ = (ryy + rzz, rx) if e0 > 0 else (rx * rx + rzz, ry)
+            u, _ = sir_fast_u(v_, w_, c1, c2, e0)
             ru = float(np.rint(u))
             if abs(u - ru) <= lim:
                 n_cert += 1
