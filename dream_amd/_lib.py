@@ -37,7 +37,7 @@ EXPORTS = (
     "dream_hip_sir_plan_status", "dream_hip_sir_plan_destroy",
     "dream_hip_das_create", "dream_hip_das_destroy", "dream_hip_das_setup_has_changed",
     "dream_hip_das_set_pixel_order", "dream_hip_pixel_order",
-    "dream_hip_das", "dream_hip_das_dev", "dream_hip_das_adjoint_dev",
+    "dream_hip_das", "dream_hip_das_dev", "dream_hip_das_status", "dream_hip_das_adjoint_dev",
     "dream_hip_das_adjoint_dev_noalloc", "dream_hip_das_time_kernels",
     "dream_hip_das_kernel_times",
     "dream_hip_das_batch_create", "dream_hip_das_batch_run", "dream_hip_das_batch_destroy",
@@ -177,6 +177,8 @@ def _declare(L):
     L.dream_hip_das_dev.restype = _i32
     L.dream_hip_das_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _dbl, _i32, _vp, _dbl, _vp, _vp,
                                     _i32, _i64, _i64, _i64, _vp, _i32, _ip, _vp]
+    L.dream_hip_das_status.restype = _i32
+    L.dream_hip_das_status.argtypes = [_vp, _i32, _ip]
     L.dream_hip_das_uni_create.restype = _i32
     L.dream_hip_das_uni_create.argtypes = [_i32, _i64, _dp, _dp, _dp, _i32, ctypes.POINTER(_vp)]
     L.dream_hip_das_uni_destroy.restype = _i32

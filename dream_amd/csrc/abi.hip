@@ -40,7 +40,7 @@ std::atomic<int64_t> g_cancel_after(0);        // > 0: abort after that many chu
 int64_t chunk_work()
 {
   const int64_t w = g_chunk_work.load(std::memory_order_relaxed);
-  return w > 0 ? w : (int64_t) 10000000000ll;
+  return w > 0 ? w : (int64_t) 50000000000ll;
 }
 
 // A gate costs nothing until a call is actually cut into chunks: the capture
@@ -665,6 +665,16 @@ int dream_hip_das_dev(void *handle, const void *Y, const void *Ro, const void *G
         Y, Ro, Gt, Gr, dt, delay_type, delay, cp, apod_t, apod_r, interp, n_frames,
         y_frame_stride, im_frame_stride, Im, err_level, sir_err != nullptr, s);
     if (sir_err) *sir_err = code;
+    return DREAM_HIP_OK;
+  });
+}
+
+int dream_hip_das_status(void *handle, int err_level, int *sir_err)
+{
+  return guarded([&]() {
+    require(handle != nullptr && sir_err != nullptr, "das status: null handle or code");
+    require(err_level >= 0 && err_level <= 4, "das: unknown error level");
+    *sir_err = static_cast<DasHandle *>(handle)->status(err_level);
     return DREAM_HIP_OK;
   });
 }

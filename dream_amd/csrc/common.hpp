@@ -59,7 +59,8 @@ struct Status {
 bool cancelled();
 
 // Work units (pixel-channel pairs, sub-element contributions) per chunk of a
-// long call: about 4 ms of device time (default 10^10), or the value a
+// long call: about 20-30 ms of device time (default 5 x 10^10; each chunk
+// launch costs a partly filled last wave of work-groups), or the value a
 // test set through dream_hip_test_chunking.
 int64_t chunk_work();
 

@@ -353,11 +353,16 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
           tt = leg3<GT>(Gt[i] - xo, gy_c, Gt[i + 2 * Lt] - zo) * inv_cp;
         if (!delay_rx) tt += dl;
         tt = clamp_leg<GT>(tt, tmax);
-        const bool row_out = !valid || !((tt + trmin) * fs < (GT) K) || !((tt + trmax) * fs > GT(-1));
-        if (__all(row_out)) {
-          if (STATUS) oob = true;
-          continue;
-        }
+        // The row's sample positions lie in [lo, hi] (rounding is monotone,
+        // so the extreme receive legs give the extreme positions).  That
+        // makes the out-of-bounds status a per-row test: a pair is out when
+        // its index trunc(u) (nearest; floor(u) when interpolating) falls
+        // outside [0, K).
+        const GT lo = (tt + trmin) * fs, hi = (tt + trmax) * fs;
+        const bool row_out = !valid || !(lo < (GT) K) || !(hi > GT(-1));
+        if (STATUS)
+          oob |= !(INTERP == DREAM_HIP_NEAREST ? lo > GT(-1) : lo >= GT(0)) || !(hi < (GT) K);
+        if (__all(row_out)) continue;
         const DT wi = APOD ? at[i] : DT(1);
         // Receivers in groups of kG (the whole chunk for nearest-lower
         // sampling, half of it for linear interpolation): every load of a
@@ -377,7 +382,6 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
             int k;
             gather_offsets<DT, GT, INTERP>((unsigned) a * Ku, Ku, (tt + tr[g0 + jj]) * fs, live,
                                            o0[jj], o1[jj], f[jj], k);
-            if (STATUS) oob |= live && (unsigned) k >= Ku;
           }
 #pragma unroll
           for (int jj = 0; jj < kG; jj++) {
@@ -426,8 +430,13 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
     else partial[((int64_t) sidx * A.n_frames + frame) * A.No + p] = sum;
   }
   if (STATUS) {
+    // Most waves of an out-of-range image find the word already raised: a
+    // device-scope load first keeps them off the one contended atomic.
     oob = oob && valid;
-    if (__any(oob) && lane == 0) atomicOr(&A.status->any_oob, 1u);
+    if (__any(oob) && lane == 0 &&
+        __hip_atomic_load(&A.status->any_oob, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+            A.status_epoch)
+      atomicMax(&A.status->any_oob, A.status_epoch);
   }
 }
 
@@ -1242,19 +1251,20 @@ void fwd_launch_shape(const DasArgs &a0, FwdShape sh, hipStream_t s, ChunkGate *
   run_chunked(tiles, a0.n_frames, 64 * pairs_per_pixel(TYPE, a0.Lt, a0.Lr), gate, launch);
 }
 
-// The heuristic shape, or the autotuner's pick for this shape class (not
-// when the launch also collects the out-of-bounds status).
+// The heuristic shape, or the autotuner's pick for this shape class (with
+// or without the out-of-bounds status).
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
 void fwd_launch(const DasArgs &a, hipStream_t s, ChunkGate *gate)
 {
   const std::vector<FwdShape> cand =
       forward_candidates(TYPE, a.No, a.n_frames, a.Lt, a.Lr, a.recip != 0);
   int pick = 0;
-  if (!a.status && cand.size() > 1) {
+  if (cand.size() > 1) {
     char key[192];
-    std::snprintf(key, sizeof key, "das_fwd|%d|%d|%d|%d|%d|%lld|%d|%d|%lld|%lld|%d", TYPE,
+    std::snprintf(key, sizeof key, "das_fwd|%d|%d|%d|%d|%d|%lld|%d|%d|%lld|%lld|%d|%d", TYPE,
                   (int) sizeof(DT), (int) sizeof(GT), INTERP, (int) APOD, (long long) a.No, a.Lt,
-                  a.Lr, (long long) a.K, (long long) a.n_frames, (int) a.recip);
+                  a.Lr, (long long) a.K, (long long) a.n_frames, (int) a.recip,
+                  (int) (a.status != nullptr));
     const int t = autotune(key, (int) cand.size(),
                            [&](int i, hipStream_t st) { fwd_launch_shape<DT, GT, TYPE, INTERP, APOD>(a, cand[(size_t) i], st); },
                            s);
@@ -1586,6 +1596,7 @@ DasHandle::DasHandle(int type, int64_t K, int64_t No, int64_t Lt, int64_t Lr, in
           "das: RCA stripe extents are read at rows[Lt .. 2 Lt - 1] and columns[Lr - 1] "
           "(extra_src/das.cc:464-467); need 2 Lt <= 3 Lr and Lr <= 3 Lt");
   DREAM_HIP_CHECK(hipMalloc(&d_status_, sizeof(Status)));
+  DREAM_HIP_CHECK(hipMemset(d_status_, 0, sizeof(Status)));
   DREAM_HIP_CHECK(hipHostMalloc(&h_status_, sizeof(Status)));
 }
 
@@ -1699,6 +1710,7 @@ void DasHandle::fill_args(DasArgs &a, const void *Y, const void *Ro, const void 
   a.im_stride = im_stride;
   a.Im = Im;
   a.status = d_status_;
+  a.status_epoch = epoch_;
   a.perm = d_perm_;
   a.scratch = d_scratch_;
   a.scratch_bytes = d_scratch_bytes_;
@@ -1727,8 +1739,13 @@ int DasHandle::run_dev(const void *Y, const void *Ro, const void *Gt, const void
   DasArgs a;
   fill_args(a, Y, Ro, Gt, Gr, dt, delay_type, delay, cp, apod_t, apod_r, n_frames, y_stride,
             im_stride, Im);
-  if (want_status) zero_async(d_status_, sizeof(Status), s);
-  else a.status = nullptr;
+  // A new epoch per call (the word is cleared once per 2^32 calls).
+  if (++epoch_ == 0) {
+    zero_async(d_status_, sizeof(Status), s);
+    epoch_ = 1;
+  }
+  a.status_epoch = epoch_;
+  status_stream_ = s;
   const bool timed = t_n_ < t_cap_ && No_ > 0;
   if (timed) DREAM_HIP_CHECK(hipEventRecord(tev_[(size_t) 3 * t_n_], s));
   // Reciprocal TFM: one array for both sides (the same pointer), one scalar
@@ -1768,9 +1785,16 @@ int DasHandle::run_dev(const void *Y, const void *Ro, const void *Gt, const void
   if (No_ > 0) launch_das_forward(a, type_, prec_, interp, apod, s, &gate);
   if (timed) DREAM_HIP_CHECK(hipEventRecord(tev_[(size_t) 3 * t_n_++ + 2], s));
   if (!want_status) return E_NONE;
+  return status(err_level);
+}
+
+int DasHandle::status(int err_level)
+{
+  if (epoch_ == 0) return E_NONE;
+  hipStream_t s = status_stream_;
   DREAM_HIP_CHECK(hipMemcpyAsync(h_status_, d_status_, sizeof(Status), hipMemcpyDeviceToHost, s));
   DREAM_HIP_CHECK(hipStreamSynchronize(s));
-  if (!h_status_->any_oob) return E_NONE;
+  if (h_status_->any_oob != epoch_) return E_NONE;
   return level_code(err_level);
 }
 

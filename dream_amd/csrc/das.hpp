@@ -26,7 +26,11 @@ struct DasArgs {
   double sx_lo, sx_hi, sy_lo, sy_hi;
   int64_t n_frames, y_stride, im_stride;
   void *Im;
+  // Forward out-of-bounds status: a pixel with an out-of-range pair raises
+  // status->any_oob to status_epoch (atomic max), so a call is told from the
+  // ones before it without zeroing the word first.
   Status *status;
+  uint32_t status_epoch;
   // Adjoint-only.
   const void *g;  void *gY;  void *z_t;  void *z_r;  void *z_cp;
   // Pixel visiting order (device, No entries) or null for 0..No-1, and the
@@ -70,8 +74,9 @@ public:
   int run_host(const void *Y, const void *Ro, const void *Gt, const void *Gr, double dt,
                int delay_type, const void *delay, double cp, void *Im, int err_level);
 
-  // Device-pointer forward; the status code is returned only when
-  // `want_status` (then the call synchronises).
+  // Device-pointer forward.  The out-of-bounds status is always computed on
+  // the device; the code is returned only when `want_status` (then the call
+  // synchronises), else status() reads it later.
   int run_dev(const void *Y, const void *Ro, const void *Gt, const void *Gr, double dt,
               int delay_type, const void *delay, double cp, const void *apod_t,
               const void *apod_r, int interp, int64_t n_frames, int64_t y_stride,
@@ -82,6 +87,10 @@ public:
                    int64_t n_frames, int64_t y_stride, int64_t im_stride, const void *g,
                    void *gY, void *g_apod_t, void *g_apod_r, void *g_cp, hipStream_t s,
                    double *fixed_z = nullptr, size_t fixed_z_bytes = 0);
+
+  // The SIRError code of the last forward call at `err_level` (waits for
+  // that call's stream).
+  int status(int err_level);
 
   int type() const { return type_; }
   int precision() const { return prec_; }
@@ -100,6 +109,8 @@ private:
   int prec_;
   Status *d_status_ = nullptr;
   Status *h_status_ = nullptr;   // pinned
+  uint32_t epoch_ = 0;           // the last forward call's status epoch
+  hipStream_t status_stream_ = nullptr;
   // Device copies for the host-pointer path (allocated on first use).
   void *d_buf_ = nullptr;
   size_t d_buf_bytes_ = 0;

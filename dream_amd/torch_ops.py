@@ -189,6 +189,14 @@ class DasPlan:
             ctypes.byref(code) if want_status else None, _stream()))
         return (Im, code.value) if want_status else Im
 
+    def status(self, err_level="ignore"):
+        """SIRError code of the last forward call (the device computes the
+        out-of-bounds status in every call; this waits for it)."""
+        code = ctypes.c_int(0)
+        _lib.check(_lib.lib().dream_hip_das_status(self.h, _lib.ERR_LEVEL[err_level],
+                                                   ctypes.byref(code)))
+        return code.value
+
     def adjoint(self, Y, g, dt, delay, cp, apod_t=None, apod_r=None, interp="linear",
                 want_gy=True, want_apod=True, want_cp=True):
         """Aᵀg: returns (gY or None, g_apod_t, g_apod_r, g_cp)."""

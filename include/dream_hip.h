@@ -88,8 +88,8 @@ void dream_hip_set_autotune(int on);
  * safe to call from a signal handler or another thread.  Every entry point
  * tests the flag on entry.  Long eager calls (DAS forwards and adjoints, SIR
  * plans and the host-pointer SIR entry points) run as consecutive chunk
- * launches of about 4 ms of device time each and test the flag before each
- * chunk, keeping at most two chunks queued ahead of the device (the calling
+ * launches of about 20-30 ms of device time each and test the flag before
+ * each chunk, keeping at most two chunks queued ahead of the device (the calling
  * thread waits for the device meanwhile), so a call cancelled while it runs
  * stops within about two chunks and returns DREAM_HIP_ECANCELLED with the
  * outputs of the launched chunks written.  Calls that capture into or replay
@@ -104,7 +104,7 @@ int dream_hip_is_running(void);
 
 /* Test hook, no counterpart in the reference: work units (pixel-channel
  * pairs or sub-element contributions) per chunk of a long call (<= 0: the
- * default, 10^10), and, when cancel_after_chunks > 0, a dream_hip_abort()
+ * default, 5 x 10^10), and, when cancel_after_chunks > 0, a dream_hip_abort()
  * issued by the library itself right after that many further chunks have
  * been launched (<= 0: off).  Lets tests run small problems as many chunks
  * and cancel a call deterministically between two of them. */
@@ -355,13 +355,23 @@ int dream_hip_das(void *handle, const void *Y, const void *Ro, const void *Gt,
  * A-scan n by apod_t[n], TFM/RCA weight pair (i,j) by apod_t[i]*apod_r[j].
  * n_frames >= 1 frames are beamformed in one launch: frame f reads
  * Y + f*y_frame_stride and writes Im + f*im_frame_stride (element strides).
- * Asynchronous; *sir_err (may be NULL) is written when the stream passes the
- * end of the call. */
+ * Every call computes the out-of-bounds status on the device, as the
+ * reference's das does (extra_src/das.cc:338-351).  With sir_err == NULL the
+ * call is asynchronous and dream_hip_das_status reads the code later; with
+ * sir_err it waits for its stream and writes the code. */
 int dream_hip_das_dev(void *handle, const void *Y, const void *Ro, const void *Gt,
                       const void *Gr, double dt, int delay_type, const void *delay,
                       double cp, const void *apod_t, const void *apod_r, int interp,
                       int64_t n_frames, int64_t y_frame_stride, int64_t im_frame_stride,
                       void *Im, int err_level, int *sir_err, void *stream);
+
+/* The SIRError code (at err_level) of the handle's last dream_hip_das_dev
+ * call: waits for that call's stream.  No counterpart in the reference,
+ * whose das returns the code synchronously; for device callers that check
+ * the status after a batch of calls.  Replays of a captured call share the
+ * call's status word, which then reports an out-of-bounds pair of any replay
+ * since the capture. */
+int dream_hip_das_status(void *handle, int err_level, int *sir_err);
 
 /* Adjoint of dream_hip_das_dev (linear or nearest rule) for the cotangent
  * g (No per frame, DEVICE).  Outputs (DEVICE, NULL to skip; all overwritten):

@@ -138,6 +138,44 @@ def test_random_data_f64_and_error_codes(gpu, oracle, method):
     assert np.max(np.abs(Im - ref)) <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("method", METHODS)
+def test_status_on_device_every_call(gpu, oracle, method):
+    """Every forward computes the out-of-bounds status on the device (the
+    reference's das always does, extra_src/das.cc:338-351); DasPlan.status
+    reads the last call's code later.  One plan, calls with and without
+    out-of-range pairs in turn: each reports its own status (a call with
+    none after one with some reports none), chunked launches report what
+    one launch does, and the synchronous code agrees."""
+    import ctypes
+
+    from dream_amd import _lib
+    from dream_amd.torch_ops import DasPlan
+    Y, Gt, Gr, Ro, _ = _setup(method, seed=2)
+    Yl = np.vstack([Y] * 3)
+    K = Yl.shape[0]
+    saft = method == "saft"
+    Yd = torch.tensor(Yl.T.copy(), device="cuda")
+    plan = DasPlan(method, K, Ro, Gt, None if saft else Gr, precision="f64")
+    seen = set()
+    for delay in (30.0, 0.0, 30.0, 0.0):
+        _, err = oracle.das(Yl, Gt, Gr, Ro, 0.02, delay, 1500.0, method, "warn", "f64",
+                            nthreads=4)
+        seen.add(err)
+        plan.forward(Yd, 0.02, delay, 1500.0)
+        assert plan.status("warn") == err, (delay, err)
+        assert plan.status("stop") == (3 if err else 0)
+        _, code = plan.forward(Yd, 0.02, delay, 1500.0, err_level="warn", want_status=True)
+        assert code == err
+    assert seen == {0, 2}
+    try:
+        _lib.lib().dream_hip_test_chunking(ctypes.c_int64(64 * 8), ctypes.c_int64(0))
+        for delay, err in ((30.0, 2), (0.0, 0)):
+            plan.forward(Yd, 0.02, delay, 1500.0)
+            assert plan.status("warn") == err
+    finally:
+        _lib.lib().dream_hip_test_chunking(ctypes.c_int64(0), ctypes.c_int64(0))
+
+
 def test_c2_tfm_f32_vs_f64_oracle(gpu, oracle):
     """The C2 TFM configuration: FP32 data / FP64 delays within 2e-5 rel-L2 of
     the FP64 oracle; FP64 within 1e-12; reference-float path against the
