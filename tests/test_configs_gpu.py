@@ -533,3 +533,22 @@ def test_unreadable_tensors_are_refused_before_launch(gpu):
         conv_sum(torch.ones((2, 8), dtype=torch.float64, device="cuda"),
                  torch.ones(3, dtype=torch.float64))                         # host filter
     torch.cuda.synchronize()
+
+
+def test_frames_of_4gib_are_refused(gpu):
+    """Gather offsets are 32-bit byte offsets inside one frame: a frame of
+    4 GiB or more (FP64 TFM, 256 x 256 elements, K = 8192) is refused by the
+    forward and the adjoint before any launch, not wrapped."""
+    from dream_amd.torch_ops import DasPlan
+    L, K = 256, 8192
+    G = np.c_[np.linspace(-32.0, 32.0, L), np.zeros(L), np.zeros(L)]
+    Ro = np.c_[np.zeros(64), np.zeros(64), np.linspace(5.0, 20.0, 64)]
+    plan = DasPlan("tfm", K, Ro, G, G, precision="f64")
+    Y = torch.empty((L * L, K), dtype=torch.float64, device="cuda")      # 4.29 GB, not read
+    with pytest.raises(RuntimeError, match="4 GiB"):
+        plan.forward(Y, 0.02, 0.0, 1500.0)
+    with pytest.raises(RuntimeError, match="4 GiB"):
+        plan.adjoint(Y, torch.ones(64, dtype=torch.float64, device="cuda"), 0.02, 0.0, 1500.0,
+                     want_gy=False)
+    del Y
+    torch.cuda.synchronize()
