@@ -1155,14 +1155,19 @@ std::vector<FwdShape> forward_candidates(int type, int64_t No, int64_t n_frames,
     if (w >= 2 && w <= kMaxWaves && std::find(waves.begin(), waves.end(), (int) w) == waves.end())
       waves.push_back((int) w);
   }
+  auto add = [&](int w, int sp) {
+    if (w > std::max<int64_t>(J, 1) || sp > 64) return;
+    if ((int64_t) w * sp > std::max<int64_t>(J, 1) * 2) return;     // most waves idle
+    for (const FwdShape &x : c)
+      if (x.waves == w && x.split == sp) return;
+    c.push_back(FwdShape{w, sp});
+  };
   for (int w : waves)
-    for (int sp : splits) {
-      if (w > std::max<int64_t>(J, 1) || sp > 64) continue;
-      if ((int64_t) w * sp > std::max<int64_t>(J, 1) * 2) continue;   // most waves idle
-      bool dup = false;
-      for (const FwdShape &x : c) dup |= (x.waves == w && x.split == sp);
-      if (!dup) c.push_back(FwdShape{w, sp});
-    }
+    for (int sp : splits) add(w, sp);
+  // One wave per work-group and eight work-groups per tile: fewer jobs per
+  // wave and more, smaller work-groups (measured faster for the FP64 and
+  // reference-float TFM images).
+  add(1, 8);
   return c;
 }
 
