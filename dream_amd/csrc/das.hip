@@ -1158,16 +1158,21 @@ std::vector<FwdShape> forward_candidates(int type, int64_t No, int64_t n_frames,
   auto add = [&](int w, int sp) {
     if (w > std::max<int64_t>(J, 1) || sp > 64) return;
     if ((int64_t) w * sp > std::max<int64_t>(J, 1) * 2) return;     // most waves idle
+    // Split partial sums (up to 8 bytes each) beyond 4 GiB are not worth
+    // the memory.
+    if (sp > 1 && (double) sp * (double) n_frames * (double) No * 8.0 > 4294967296.0) return;
     for (const FwdShape &x : c)
       if (x.waves == w && x.split == sp) return;
     c.push_back(FwdShape{w, sp});
   };
   for (int w : waves)
     for (int sp : splits) add(w, sp);
-  // One wave per work-group and eight work-groups per tile: fewer jobs per
-  // wave and more, smaller work-groups (measured faster for the FP64 and
-  // reference-float TFM images).
+  // Small work-groups and many per tile: fewer jobs per wave and more,
+  // smaller work-groups (measured faster for the FP64 and reference-float
+  // TFM images and the eight-frame RCA call).
   add(1, 8);
+  add(1, 16);
+  add(2, 4);
   return c;
 }
 
