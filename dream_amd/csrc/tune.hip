@@ -113,16 +113,27 @@ int autotune(const std::string &key, int n, const std::function<void(int, hipStr
   DREAM_HIP_CHECK(hipEventCreate(&e0));
   DREAM_HIP_CHECK(hipEventCreate(&e1));
   int best = 0;
-  float best_ms = 0.0f;
+  float best_ms = 0.0f;                           // per launch
   try {
     for (int i = 0; i < n; i++) {
-      run(i, s);                                  // warm-up
       DREAM_HIP_CHECK(hipEventRecord(e0, s));
-      for (int r = 0; r < kReps; r++) run(i, s);
+      run(i, s);                                  // warm-up
+      DREAM_HIP_CHECK(hipEventRecord(e1, s));
+      DREAM_HIP_CHECK(hipEventSynchronize(e1));
+      float warm = 0.0f;
+      DREAM_HIP_CHECK(hipEventElapsedTime(&warm, e0, e1));
+      // Long launches (tens of ms, where a first launch's set-up is noise)
+      // are timed once, and not at all past 1.5 times the best so far.
+      const bool long_run = warm > 20.0f;
+      if (long_run && i > 0 && warm > 1.5f * best_ms) continue;
+      const int reps = long_run ? 1 : kReps;
+      DREAM_HIP_CHECK(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; r++) run(i, s);
       DREAM_HIP_CHECK(hipEventRecord(e1, s));
       DREAM_HIP_CHECK(hipEventSynchronize(e1));
       float ms = 0.0f;
       DREAM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      ms /= (float) reps;
       if (i == 0 || ms < best_ms) {
         best = i;
         best_ms = ms;
