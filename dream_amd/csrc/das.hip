@@ -116,6 +116,40 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(const void *base, i
                                            0x00020000);
 }
 
+// The forward reads one A-scan at a time through a structured view of the
+// frame: K records of sizeof(DT) bytes, with the A-scan's byte offset a K
+// sizeof(DT) (< 4 GiB, the frame limit) in the scalar offset, which the
+// hardware adds after its range check.  The check (record index < K) is
+// then the bounds test of the sample index itself: an index outside [0, K),
+// negative ones included (|k| <= 2^28, so they wrap to >= 2^32 - 2^28 > K),
+// reads 0 without touching memory, and a gather costs no more than the
+// conversion of its position (a raw view cannot do this: its range check
+// covers the scalar offset too).  kNoRecord marks a pair that is not live.
+constexpr unsigned kNoRecord = 0xFFFFFFFFu;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ascan_rsrc(const void *frame, int K, int elem)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(frame), (short) elem, K,
+                                           0x00020000);
+}
+
+extern "C" __device__ float das_struct_load_f32(__amdgpu_buffer_rsrc_t r, int index, int voffset,
+                                                int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.f32");
+extern "C" __device__ double das_struct_load_f64(__amdgpu_buffer_rsrc_t r, int index, int voffset,
+                                                 int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.f64");
+
+// Sample `index` of the A-scan at byte offset `soff` (0 outside [0, K)).
+template <typename DT>
+__device__ __forceinline__ DT sload(__amdgpu_buffer_rsrc_t r, unsigned index, unsigned soff)
+{
+  if constexpr (std::is_same<DT, float>::value)
+    return das_struct_load_f32(r, (int) index, 0, (int) soff, 0);
+  else
+    return das_struct_load_f64(r, (int) index, 0, (int) soff, 0);
+}
+
 template <typename DT>
 __device__ __forceinline__ DT bload(__amdgpu_buffer_rsrc_t r, unsigned byte_off);
 template <>
@@ -179,31 +213,29 @@ template <typename DT, typename GT, int INTERP>
 constexpr bool kSaftFast = std::is_same<GT, double>::value &&
                            (INTERP == DREAM_HIP_NEAREST || sizeof(DT) == 4);
 
-// Forward gathers go through one buffer descriptor per frame.  The lane's
-// byte offset is (a*K + k) * sizeof(DT) when the sample index k lies in
-// [0, K), and kNoSample (past the frame, read as 0 by the hardware range
-// check) otherwise: one compare and one select per gather, and the whole
-// launch shares one descriptor (per-A-scan descriptors would need four
-// scalar registers each and spill when a row keeps 16 loads in flight).
-// Positions arrive with |u| < 2^28 (legs clamped to leg_max = 2^26 / fs, or
-// a certified SAFT position), so the integer conversion is exact.  A pair
-// that is not `live` (padding past the last
-// element of a block) reads 0 through the same unconditional load, so the
-// loads of a block never wait on a branch.  Offsets (and the linear weight)
-// come first; callers issue a block's loads together and combine after.
-template <typename DT, typename GT, int INTERP>
-__device__ __forceinline__ void gather_offsets(unsigned abase, unsigned K, GT u, bool live,
-                                               unsigned &o0, unsigned &o1, DT &f, int &k)
+// Forward gathers go through the frame's structured view (ascan_rsrc): the
+// record index is the sample index k itself and the A-scan's offset is
+// scalar, so the hardware range check does the bounds test.  Positions
+// arrive with |u| < 2^28 (legs clamped to leg_max = 2^26 / fs, or a
+// certified SAFT position), so the integer conversion is exact.  A pair
+// that is not `live` (padding past the last element of a block, or the
+// lower half of a diagonal reciprocal block) gets kNoRecord and reads 0
+// through the same unconditional load, so the loads of a block never wait
+// on a branch; with ALL_LIVE (a block without such pairs) the select is
+// left out.  Indices (and the linear weight) come first; callers issue a
+// block's loads together and combine after.
+template <typename DT, typename GT, int INTERP, bool ALL_LIVE>
+__device__ __forceinline__ void gather_index(GT u, bool live, unsigned &i0, unsigned &i1, DT &f,
+                                             int &k)
 {
-  constexpr unsigned s = (unsigned) sizeof(DT);
   if (INTERP == DREAM_HIP_NEAREST) {
     k = (int) u;
-    o0 = (live && (unsigned) k < K) ? (abase + (unsigned) k) * s : kNoSample;
+    i0 = (ALL_LIVE || live) ? (unsigned) k : kNoRecord;
   } else {
     const GT fk = floor(u);
     k = (int) fk;
-    o0 = (live && (unsigned) k < K) ? (abase + (unsigned) k) * s : kNoSample;
-    o1 = (live && (unsigned) (k + 1) < K) ? (abase + (unsigned) (k + 1)) * s : kNoSample;
+    i0 = (ALL_LIVE || live) ? (unsigned) k : kNoRecord;
+    i1 = (ALL_LIVE || live) ? (unsigned) (k + 1) : kNoRecord;
     f = (DT) (u - fk);
   }
 }
@@ -247,7 +279,8 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
   if (RECIP && TYPE == DAS_TFM)
     recip = __builtin_amdgcn_readfirstlane((int) (delay_ms_of<GT>(gdel[0]) == GT(0))) != 0;
   const DT *Yf = (const DT *) (recip ? A.Ysym : A.Y) + frame * A.y_stride;
-  const __amdgpu_buffer_rsrc_t rY = frame_rsrc(Yf, A.y_stride * (int64_t) sizeof(DT));
+  const __amdgpu_buffer_rsrc_t rY = ascan_rsrc(Yf, K, (int) sizeof(DT));
+  const unsigned KB = Ku * (unsigned) sizeof(DT);          // bytes per A-scan
   const GT *Gt = (const GT *) A.Gt;
   const GT *Gr = (const GT *) A.Gr;
   const DT *at = (const DT *) A.apod_t;
@@ -281,16 +314,16 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
       DT f[kSB], y0[kSB], y1[kSB];
 #pragma unroll
       for (int q = 0; q < kSB; q++) {
-        const int n = min(n0 + q, Lt - 1);
         const bool live = n0 + q < Lt;
         int k;
-        gather_offsets<DT, GT, INTERP>((unsigned) n * Ku, Ku, u[q], live, o0[q], o1[q], f[q], k);
+        gather_index<DT, GT, INTERP, false>(u[q], live, o0[q], o1[q], f[q], k);
         if (STATUS) oob |= live && (unsigned) k >= Ku;
       }
 #pragma unroll
       for (int q = 0; q < kSB; q++) {
-        y0[q] = bload<DT>(rY, o0[q]);
-        if (INTERP != DREAM_HIP_NEAREST) y1[q] = bload<DT>(rY, o1[q]);
+        const unsigned soff = (unsigned) min(n0 + q, Lt - 1) * KB;
+        y0[q] = sload<DT>(rY, o0[q], soff);
+        if (INTERP != DREAM_HIP_NEAREST) y1[q] = sload<DT>(rY, o1[q], soff);
       }
       DT part = DT(0);
 #pragma unroll
@@ -345,57 +378,68 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
         trmin = fmin(trmin, tr[jj]);       // clamped duplicates of the last
         trmax = fmax(trmax, tr[jj]);       // receiver do not widen the range
       }
-      for (int i = i0; i < i1; i++) {
-        GT tt;
-        if (TYPE == DAS_TFM)
-          tt = leg3<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
-        else
-          tt = leg3<GT>(Gt[i] - xo, gy_c, Gt[i + 2 * Lt] - zo) * inv_cp;
-        if (!delay_rx) tt += dl;
-        tt = clamp_leg<GT>(tt, tmax);
-        // The row's sample positions lie in [lo, hi] (rounding is monotone,
-        // so the extreme receive legs give the extreme positions).  That
-        // makes the out-of-bounds status a per-row test: a pair is out when
-        // its index trunc(u) (nearest; floor(u) when interpolating) falls
-        // outside [0, K).
-        const GT lo = (tt + trmin) * fs, hi = (tt + trmax) * fs;
-        const bool row_out = !valid || !(lo < (GT) K) || !(hi > GT(-1));
-        if (STATUS)
-          oob |= !(INTERP == DREAM_HIP_NEAREST ? lo > GT(-1) : lo >= GT(0)) || !(hi < (GT) K);
-        if (__all(row_out)) continue;
-        const DT wi = APOD ? at[i] : DT(1);
-        // Receivers in groups of kG (the whole chunk for nearest-lower
-        // sampling, half of it for linear interpolation): every load of a
-        // group is issued before any is used.  Receivers past Lr (in the
-        // last chunk) read 0.
-        constexpr int kG = INTERP == DREAM_HIP_NEAREST ? kRC : kRC / 2;
-        DT part = DT(0);
+      // Rows of the job; ALL_LIVE (every pair of the block is live: a full
+      // chunk off the diagonal) leaves the per-pair live select out.
+      auto rows = [&](auto all_live) {
+        constexpr bool ALL_LIVE = decltype(all_live)::value;
+        for (int i = i0; i < i1; i++) {
+          GT tt;
+          if (TYPE == DAS_TFM)
+            tt = leg3<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
+          else
+            tt = leg3<GT>(Gt[i] - xo, gy_c, Gt[i + 2 * Lt] - zo) * inv_cp;
+          if (!delay_rx) tt += dl;
+          tt = clamp_leg<GT>(tt, tmax);
+          // The row's sample positions lie in [lo, hi] (rounding is monotone,
+          // so the extreme receive legs give the extreme positions).  That
+          // makes the out-of-bounds status a per-row test: a pair is out when
+          // its index trunc(u) (nearest; floor(u) when interpolating) falls
+          // outside [0, K).
+          const GT lo = (tt + trmin) * fs, hi = (tt + trmax) * fs;
+          const bool row_out = !valid || !(lo < (GT) K) || !(hi > GT(-1));
+          if (STATUS)
+            oob |= !(INTERP == DREAM_HIP_NEAREST ? lo > GT(-1) : lo >= GT(0)) || !(hi < (GT) K);
+          if (__all(row_out)) continue;
+          const DT wi = APOD ? at[i] : DT(1);
+          // Receivers in groups of kG (the whole chunk for nearest-lower
+          // sampling, half of it for linear interpolation): every load of a
+          // group is issued before any is used.  Receivers past Lr (in the
+          // last chunk) read 0.
+          constexpr int kG = INTERP == DREAM_HIP_NEAREST ? kRC : kRC / 2;
+          DT part = DT(0);
 #pragma unroll
-        for (int g0 = 0; g0 < kRC; g0 += kG) {
-          unsigned o0[kG], o1[kG];
-          DT f[kG], y0[kG], y1[kG];
+          for (int g0 = 0; g0 < kRC; g0 += kG) {
+            unsigned o0[kG], o1[kG];
+            DT f[kG], y0[kG], y1[kG];
 #pragma unroll
-          for (int jj = 0; jj < kG; jj++) {
-            const int j = min(j0 + g0 + jj, Lr - 1);
-            const bool live = (full || j0 + g0 + jj < Lr) && (!diag || j0 + g0 + jj >= i);
-            const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
-            int k;
-            gather_offsets<DT, GT, INTERP>((unsigned) a * Ku, Ku, (tt + tr[g0 + jj]) * fs, live,
-                                           o0[jj], o1[jj], f[jj], k);
+            for (int jj = 0; jj < kG; jj++) {
+              const bool live = (full || j0 + g0 + jj < Lr) && (!diag || j0 + g0 + jj >= i);
+              int k;
+              gather_index<DT, GT, INTERP, ALL_LIVE>((tt + tr[g0 + jj]) * fs, live, o0[jj], o1[jj],
+                                                     f[jj], k);
+            }
+#pragma unroll
+            for (int jj = 0; jj < kG; jj++) {
+              const int j = min(j0 + g0 + jj, Lr - 1);
+              const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
+              const unsigned soff = (unsigned) a * KB;
+              y0[jj] = sload<DT>(rY, o0[jj], soff);
+              if (INTERP != DREAM_HIP_NEAREST) y1[jj] = sload<DT>(rY, o1[jj], soff);
+            }
+#pragma unroll
+            for (int jj = 0; jj < kG; jj++) {
+              const DT v =
+                  INTERP == DREAM_HIP_NEAREST ? y0[jj] : y0[jj] + f[jj] * (y1[jj] - y0[jj]);
+              part += APOD ? ar[min(j0 + g0 + jj, Lr - 1)] * v : v;
+            }
           }
-#pragma unroll
-          for (int jj = 0; jj < kG; jj++) {
-            y0[jj] = bload<DT>(rY, o0[jj]);
-            if (INTERP != DREAM_HIP_NEAREST) y1[jj] = bload<DT>(rY, o1[jj]);
-          }
-#pragma unroll
-          for (int jj = 0; jj < kG; jj++) {
-            const DT v = INTERP == DREAM_HIP_NEAREST ? y0[jj] : y0[jj] + f[jj] * (y1[jj] - y0[jj]);
-            part += APOD ? ar[min(j0 + g0 + jj, Lr - 1)] * v : v;
-          }
+          acc += APOD ? wi * part : part;
         }
-        acc += APOD ? wi * part : part;
-      }
+      };
+      if (full && !diag)
+        rows(std::true_type{});
+      else
+        rows(std::false_type{});
     }
   }
   return acc;
@@ -430,13 +474,15 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
     else partial[((int64_t) sidx * A.n_frames + frame) * A.No + p] = sum;
   }
   if (STATUS) {
-    // Most waves of an out-of-range image find the word already raised: a
-    // device-scope load first keeps them off the one contended atomic.
+    // Every writer of a call stores the same value, the call's epoch, so a
+    // plain store does it: it stays in the writing XCD's L2 until the
+    // kernel ends, where an atomic (or a device-scope load) from thousands
+    // of waves would queue at one memory address -- up to a fifth of a
+    // short launch when the waves of an out-of-range image end together.
+    // Calls of one handle are stream-ordered, so a later call never sees
+    // an earlier call's store land after its own.
     oob = oob && valid;
-    if (__any(oob) && lane == 0 &&
-        __hip_atomic_load(&A.status->any_oob, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
-            A.status_epoch)
-      atomicMax(&A.status->any_oob, A.status_epoch);
+    if (__any(oob) && lane == 0) A.status->any_oob = A.status_epoch;
   }
 }
 
@@ -1173,6 +1219,13 @@ std::vector<FwdShape> forward_candidates(int type, int64_t No, int64_t n_frames,
   add(1, 8);
   add(1, 16);
   add(2, 4);
+  // One or two waves per work-group with as many work-groups per tile as
+  // deal the jobs evenly, k jobs per wave.
+  for (int64_t k = 1; k <= 3; k++) {
+    const int64_t per = (J + k - 1) / k;
+    for (int w = 1; w <= 2; w++)
+      if (per % w == 0) add(w, (int) (per / w));
+  }
   return c;
 }
 
@@ -1743,6 +1796,7 @@ int DasHandle::run_dev(const void *Y, const void *Ro, const void *Gt, const void
   const bool apod = apod_t || apod_r;
   require(!apod || apod_t, "das: apod_r given without apod_t");
   require(!apod || type_ == DAS_SAFT || apod_r, "das: TFM/RCA apodization needs apod_r");
+  require(y_stride >= n_ascans * K_, "das: Y frames must not overlap (stride >= K x A-scans)");
   require((uint64_t) y_stride * elem_size(prec_) < 0xFFFFFFF0ull,
           "das: one frame of Y must be smaller than 4 GiB");
   ensure_scratch(das_forward_scratch_bytes(type_, prec_, No_, n_frames, Lt_, Lr_));
@@ -1889,6 +1943,8 @@ void DasHandle::adjoint_dev(const void *Y, const void *Ro, const void *Gt, const
   const int64_t n_ascans = type_ == DAS_SAFT ? Lt_ : Lt_ * Lr_;
   if (y_stride <= 0) y_stride = n_ascans * K_;
   if (im_stride <= 0) im_stride = No_;
+  require(y_stride >= n_ascans * K_,
+          "das adjoint: Y frames must not overlap (stride >= K x A-scans)");
   require((uint64_t) y_stride * elem_size(prec_) < 0xFFFFFFF0ull,
           "das adjoint: one frame of Y must be smaller than 4 GiB");
   require(type_ == DAS_SAFT || gY || (size_t) (Lt_ + Lr_) * 8 <= 65536,

@@ -26,9 +26,9 @@ struct DasArgs {
   double sx_lo, sx_hi, sy_lo, sy_hi;
   int64_t n_frames, y_stride, im_stride;
   void *Im;
-  // Forward out-of-bounds status: a pixel with an out-of-range pair raises
-  // status->any_oob to status_epoch (atomic max), so a call is told from the
-  // ones before it without zeroing the word first.
+  // Forward out-of-bounds status: a pixel with an out-of-range pair sets
+  // status->any_oob to status_epoch (a new value per call), so a call is
+  // told from the ones before it without zeroing the word first.
   Status *status;
   uint32_t status_epoch;
   // Adjoint-only.

@@ -27,7 +27,7 @@
 //
 // Sample index.  The index must equal the reference's FP64 sequence bit for
 // bit.  A fast estimate u = r*c1 - d/dt (r from the hardware rsqrt, about
-// 2^-24, and one Goldschmidt step, about 2^-47; c1 = 1000/(cp dt)) differs
+// 2^-24, and one Newton step, about 2^-47; c1 = 1000/(cp dt)) differs
 // from the reference's pre-rounding value by less than 2^-44 (|u| + |d/dt|
 // + 1), against a certification margin of 2^-36 (same terms); rint(u) is
 // taken when u lies farther than B = 2^-36 (nt + 2 + |d/dt|) from a

@@ -2,6 +2,7 @@
 
 #include "tune.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -108,37 +109,49 @@ int autotune(const std::string &key, int n, const std::function<void(int, hipStr
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return -1;
 
-  constexpr int kReps = 5;
+  // Short launches are timed in kRounds interleaved rounds of enough
+  // back-to-back launches to last about kRoundMs, and each candidate keeps its fastest round, so clock
+  // ramps and other transients early in the sweep do not decide the pick.  Long launches (tens of ms, where a first
+  // launch's set-up is noise) are timed once, and not at all past 1.5 times
+  // the best long time so far.
+  constexpr int kRounds = 3, kMinReps = 5, kMaxReps = 64;
+  constexpr float kRoundMs = 1.0f, kLongMs = 20.0f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   DREAM_HIP_CHECK(hipEventCreate(&e0));
   DREAM_HIP_CHECK(hipEventCreate(&e1));
+  auto timed = [&](int i, int reps) {
+    DREAM_HIP_CHECK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) run(i, s);
+    DREAM_HIP_CHECK(hipEventRecord(e1, s));
+    DREAM_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    DREAM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    return ms / (float) reps;
+  };
+  const float kNone = -1.0f;
+  std::vector<float> ms(n, kNone);              // per launch; kNone: not measured
+  std::vector<int> reps(n, 0);                  // > 0: short candidate
   int best = 0;
-  float best_ms = 0.0f;                           // per launch
   try {
+    float best_long = kNone;
     for (int i = 0; i < n; i++) {
-      DREAM_HIP_CHECK(hipEventRecord(e0, s));
-      run(i, s);                                  // warm-up
-      DREAM_HIP_CHECK(hipEventRecord(e1, s));
-      DREAM_HIP_CHECK(hipEventSynchronize(e1));
-      float warm = 0.0f;
-      DREAM_HIP_CHECK(hipEventElapsedTime(&warm, e0, e1));
-      // Long launches (tens of ms, where a first launch's set-up is noise)
-      // are timed once, and not at all past 1.5 times the best so far.
-      const bool long_run = warm > 20.0f;
-      if (long_run && i > 0 && warm > 1.5f * best_ms) continue;
-      const int reps = long_run ? 1 : kReps;
-      DREAM_HIP_CHECK(hipEventRecord(e0, s));
-      for (int r = 0; r < reps; r++) run(i, s);
-      DREAM_HIP_CHECK(hipEventRecord(e1, s));
-      DREAM_HIP_CHECK(hipEventSynchronize(e1));
-      float ms = 0.0f;
-      DREAM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-      ms /= (float) reps;
-      if (i == 0 || ms < best_ms) {
-        best = i;
-        best_ms = ms;
+      const float warm = timed(i, 1);           // warm-up
+      if (warm > kLongMs) {
+        if (best_long >= 0.0f && warm > 1.5f * best_long) continue;
+        ms[i] = timed(i, 1);
+        if (best_long < 0.0f || ms[i] < best_long) best_long = ms[i];
+      } else {
+        reps[i] = std::min(kMaxReps, std::max(kMinReps, (int) (kRoundMs / std::max(warm, 1e-3f))));
       }
     }
+    for (int round = 0; round < kRounds; round++)
+      for (int i = 0; i < n; i++)
+        if (reps[i] > 0) {
+          const float t = timed(i, reps[i]);
+          if (ms[i] < 0.0f || t < ms[i]) ms[i] = t;
+        }
+    for (int i = 0; i < n; i++)
+      if (ms[i] >= 0.0f && (ms[best] < 0.0f || ms[i] < ms[best])) best = i;
   } catch (...) {
     (void) hipEventDestroy(e0);
     (void) hipEventDestroy(e1);

@@ -2,8 +2,9 @@
 //
 // A kernel with several valid launch shapes registers them as numbered
 // candidates.  The first launch of a shape class times every candidate on
-// the caller's stream (one warm-up and a few timed launches each, HIP
-// events) and keeps the fastest; later launches of the class reuse it.
+// the caller's stream (a warm-up, then interleaved rounds of timed
+// launches, HIP events) and keeps the fastest; later launches of the class
+// reuse it.
 // Results are cached per process, keyed by the device architecture and the
 // shape class, and -- when DREAM_HIP_TUNE_CACHE names a file -- read from and
 // appended to that file.  Tuning is skipped (the caller's default shape is

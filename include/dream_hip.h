@@ -358,7 +358,9 @@ int dream_hip_das(void *handle, const void *Y, const void *Ro, const void *Gt,
  * Every call computes the out-of-bounds status on the device, as the
  * reference's das does (extra_src/das.cc:338-351).  With sir_err == NULL the
  * call is asynchronous and dream_hip_das_status reads the code later; with
- * sir_err it waits for its stream and writes the code. */
+ * sir_err it waits for its stream and writes the code.  Calls on one handle
+ * must be ordered (one stream, or streams synchronised between the calls):
+ * they share the handle's work buffers and status word. */
 int dream_hip_das_dev(void *handle, const void *Y, const void *Ro, const void *Gt,
                       const void *Gr, double dt, int delay_type, const void *delay,
                       double cp, const void *apod_t, const void *apod_r, int interp,
