@@ -500,24 +500,54 @@ __global__ void das_reduce_kernel(const DT *__restrict__ partial, int split, int
   const int64_t f = frame0 + q / n_idx, idx = idx0 + q % n_idx;
   if (idx >= No) return;
   const int64_t p = pixel_of(perm, idx);
-  const int64_t at = f * No + p;
-  DT sum = partial[at];
-  for (int s = 1; s < split; s++) sum += partial[(int64_t) s * n_frames * No + at];
+  const int64_t at = f * No + p, stride = n_frames * No;
+  // Same order of additions as a plain loop; the loads of four partial
+  // sums are issued together.
+  DT sum;
+  int s;
+  if (split >= 4) {
+    const DT a = partial[at], b = partial[stride + at];
+    const DT c = partial[2 * stride + at], d = partial[3 * stride + at];
+    sum = a;
+    sum += b;
+    sum += c;
+    sum += d;
+    s = 4;
+  } else {
+    sum = partial[at];
+    s = 1;
+  }
+  for (; s + 4 <= split; s += 4) {
+    const DT a = partial[s * stride + at], b = partial[(s + 1) * stride + at];
+    const DT c = partial[(s + 2) * stride + at], d = partial[(s + 3) * stride + at];
+    sum += a;
+    sum += b;
+    sum += c;
+    sum += d;
+  }
+  for (; s < split; s++) sum += partial[s * stride + at];
   Im[f * im_stride + p] = sum;
 }
 
 // Ysym[f][i L + j] = Y[f][i L + j] + Y[f][j L + i] (i < j), Y[f][i L + i]
 // (i == j), for the reciprocal TFM forward; nothing when the scalar delay
 // read on the device is not zero (the forward then reads Y).  One
-// work-group per (j, i, frame); i > j exit at once.  16-byte accesses when
-// K and the frame stride keep every A-scan 16-byte aligned.
+// work-group per pair i <= j (blockIdx.x runs over the L (L + 1) / 2 pairs
+// column by column) and frame.  16-byte accesses when K and the frame
+// stride keep every A-scan 16-byte aligned, two per thread and operand
+// issued before either is used.
 template <typename DT, typename GT>
 __global__ void __launch_bounds__(256)
 das_symmetrize_kernel(const DT *__restrict__ Y, DT *__restrict__ Ys, int L, int64_t K,
                       int64_t y_stride, const GT *__restrict__ delay)
 {
-  const int j = blockIdx.x, i = blockIdx.y;
-  if (i > j || delay_ms_of<GT>(delay[0]) != GT(0)) return;
+  if (delay_ms_of<GT>(delay[0]) != GT(0)) return;
+  // Pair t = j (j + 1) / 2 + i, 0 <= i <= j < L.
+  const int t = (int) blockIdx.x;
+  int j = (int) ((sqrtf(8.0f * (float) t + 1.0f) - 1.0f) * 0.5f);
+  while ((j + 1) * (j + 2) / 2 <= t) j++;
+  while (j * (j + 1) / 2 > t) j--;
+  const int i = t - j * (j + 1) / 2;
   const int64_t f = blockIdx.z;
   const DT *a = Y + f * y_stride + ((int64_t) i * L + j) * K;
   const DT *b = Y + f * y_stride + ((int64_t) j * L + i) * K;
@@ -525,23 +555,31 @@ das_symmetrize_kernel(const DT *__restrict__ Y, DT *__restrict__ Ys, int L, int6
   constexpr int V = 16 / sizeof(DT);
   const bool vec = (K % V) == 0 && (y_stride % V) == 0 &&
                    ((reinterpret_cast<uintptr_t>(Y) | reinterpret_cast<uintptr_t>(Ys)) & 15) == 0;
+  const bool diag = i == j;
+  const int64_t step = blockDim.x;
   if (vec) {
     using V4 = typename std::conditional<sizeof(DT) == 4, float4, double2>::type;
     const V4 *av = reinterpret_cast<const V4 *>(a);
     const V4 *bv = reinterpret_cast<const V4 *>(b);
     V4 *ov = reinterpret_cast<V4 *>(o);
     const int64_t n = K / V;
-    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
-      V4 x = av[k];
-      if (i != j) x = x + bv[k];
-      ov[k] = x;
+    for (int64_t k = threadIdx.x; k < n; k += 2 * step) {
+      const bool two = k + step < n;
+      V4 x0 = av[k], x1 = two ? av[k + step] : V4{};
+      if (!diag) {
+        const V4 y0 = bv[k], y1 = two ? bv[k + step] : V4{};
+        x0 = x0 + y0;
+        x1 = x1 + y1;
+      }
+      ov[k] = x0;
+      if (two) ov[k + step] = x1;
     }
     return;
   }
-  if (i == j) {
-    for (int64_t k = threadIdx.x; k < K; k += blockDim.x) o[k] = a[k];
+  if (diag) {
+    for (int64_t k = threadIdx.x; k < K; k += step) o[k] = a[k];
   } else {
-    for (int64_t k = threadIdx.x; k < K; k += blockDim.x) o[k] = a[k] + b[k];
+    for (int64_t k = threadIdx.x; k < K; k += step) o[k] = a[k] + b[k];
   }
 }
 
@@ -1799,7 +1837,7 @@ int DasHandle::run_dev(const void *Y, const void *Ro, const void *Gt, const void
   require(y_stride >= n_ascans * K_, "das: Y frames must not overlap (stride >= K x A-scans)");
   require((uint64_t) y_stride * elem_size(prec_) < 0xFFFFFFF0ull,
           "das: one frame of Y must be smaller than 4 GiB");
-  ensure_scratch(das_forward_scratch_bytes(type_, prec_, No_, n_frames, Lt_, Lr_));
+  ensure_scratch_for(n_frames);
   DasArgs a;
   fill_args(a, Y, Ro, Gt, Gr, dt, delay_type, delay, cp, apod_t, apod_r, n_frames, y_stride,
             im_stride, Im);
@@ -1827,7 +1865,7 @@ int DasHandle::run_dev(const void *Y, const void *Ro, const void *Gt, const void
         throw Error(DREAM_HIP_ENOMEM, "das: device allocation failed (symmetrized A-scans)");
       d_sym_bytes_ = bytes;
     }
-    const dim3 grid((unsigned) Lt_, (unsigned) Lt_, (unsigned) n_frames);
+    const dim3 grid((unsigned) (Lt_ * (Lt_ + 1) / 2), 1u, (unsigned) n_frames);
     if (prec_ == DREAM_HIP_F64)
       hipLaunchKernelGGL((das_symmetrize_kernel<double, double>), grid, dim3(256), 0, s,
                          (const double *) Y, (double *) d_sym_, (int) Lt_, K_, y_stride,

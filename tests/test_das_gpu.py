@@ -496,9 +496,10 @@ def test_reciprocal_tfm_equals_full_pair_sum(gpu, oracle, prec):
     and the out-of-bounds status; a non-zero delay reads Y itself."""
     from dream_amd.torch_ops import DasPlan
     dt_ = torch.float64 if prec == "f64" else torch.float32
-    for L, No in ((64, 3000), (24, 20000)):              # split launch; unsplit, ragged blocks
+    # Split launch; unsplit, ragged blocks; an A-scan length that is not a
+    # multiple of four (scalar symmetrizing).  Some pairs fall past K.
+    for L, No, K in ((64, 3000, 1200), (24, 20000, 1200), (7, 500, 1201)):
         rng = np.random.default_rng(81 + L)
-        K = 1200                                          # some pairs fall past K
         x = np.linspace(-8.0, 8.0, L)
         G = np.c_[x, np.zeros(L), np.zeros(L)]
         Ro = np.c_[rng.uniform(-10, 10, No), np.zeros(No), rng.uniform(1, 12, No)]
