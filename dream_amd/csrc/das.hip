@@ -57,6 +57,25 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kRC = 16;          // receive legs held in registers per chunk
 
+// Correctly rounded sqrt(s) for s in [2^-767, inf): the hardware rsqrt
+// seed, one Goldschmidt step for the root g and half-reciprocal h, then two
+// Newton corrections of g from the residual s - g^2 -- the operations of
+// the compiler's IEEE square root, which only adds a scaling of arguments
+// below 2^-767 (so that the residual stays normal) and passes 0 and inf
+// through.  Five operations fewer per root.
+__device__ __forceinline__ double sqrt_normal(double s)
+{
+  const double y = __builtin_amdgcn_rsq(s);
+  double g = s * y, h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, s);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, s);
+  return __builtin_fma(d, h, g);
+}
+
 template <typename GT>
 __device__ __forceinline__ GT leg3(GT dx, GT dy, GT dz)
 {
@@ -66,6 +85,26 @@ template <>
 __device__ __forceinline__ float leg3<float>(float dx, float dy, float dz)
 {
   return __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+// leg3 for code that the whole wave runs together (the two-leg forward's
+// legs): the same value, by sqrt_normal where it applies.  Zero, tiny,
+// infinite or NaN squared distances (a pixel on an element, or non-finite
+// input) take the compiler's square root; the test is wave-wide, so the
+// common case runs no branch body.
+template <typename GT>
+__device__ __forceinline__ GT leg3_wave(GT dx, GT dy, GT dz)
+{
+  if constexpr (std::is_same<GT, double>::value) {
+    const double s = dx * dx + dy * dy + dz * dz;
+    double r = sqrt_normal(s);
+    const bool odd = !(s >= 0x1p-767 && s < __builtin_inf());
+    if (__builtin_expect(__any(odd), 0))
+      if (odd) r = __builtin_sqrt(s);
+    return r;
+  } else {
+    return leg3<GT>(dx, dy, dz);
+  }
 }
 
 // Stripe-clamped lateral distance (extra_src/das.cc:477-485).
@@ -363,9 +402,9 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
       for (int jj = 0; jj < kRC; jj++) {
         const int j = min(j0 + jj, Lr - 1);
         if (TYPE == DAS_TFM)
-          tr[jj] = leg3<GT>(Gr[j] - xo, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
+          tr[jj] = leg3_wave<GT>(Gr[j] - xo, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
         else
-          tr[jj] = leg3<GT>(gx_r, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
+          tr[jj] = leg3_wave<GT>(gx_r, Gr[j + Lr] - yo, Gr[j + 2 * Lr] - zo) * inv_cp;
         if (delay_rx) tr[jj] += dl;
         tr[jj] = clamp_leg<GT>(tr[jj], tmax);
       }
@@ -385,9 +424,9 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
         for (int i = i0; i < i1; i++) {
           GT tt;
           if (TYPE == DAS_TFM)
-            tt = leg3<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
+            tt = leg3_wave<GT>(Gt[i] - xo, Gt[i + Lt] - yo, Gt[i + 2 * Lt] - zo) * inv_cp;
           else
-            tt = leg3<GT>(Gt[i] - xo, gy_c, Gt[i + 2 * Lt] - zo) * inv_cp;
+            tt = leg3_wave<GT>(Gt[i] - xo, gy_c, Gt[i + 2 * Lt] - zo) * inv_cp;
           if (!delay_rx) tt += dl;
           tt = clamp_leg<GT>(tt, tmax);
           // The row's sample positions lie in [lo, hi] (rounding is monotone,

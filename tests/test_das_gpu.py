@@ -64,6 +64,10 @@ def test_ramp_indices_exact_all_precisions(gpu, oracle, method):
     from dream_amd import das, das_f
     from dream_amd.torch_ops import DasPlan
     _, Gt, Gr, Ro, rng = _setup(method, K=900, No=3000, seed=1)
+    # Pixels on elements (a zero squared distance) and 1e-120 from one (a
+    # squared distance below the range of the forward's fast square root).
+    Ro[0], Ro[1], Ro[2] = Gt[0], Gt[4], Gr[2]
+    Ro[3] = Gt[1] + np.array([1e-120, 0.0, 0.0])
     n_ascans = Gt.shape[0] if method == "saft" else Gt.shape[0] * Gr.shape[0]
     ramp = np.repeat(np.arange(900, dtype=np.float64)[:, None], n_ascans, axis=1)
     for delay in (0.0, 0.31, rng.uniform(-0.4, 0.9, Ro.shape[0])):
