@@ -1067,8 +1067,10 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
     const int64_t p = valid ? pixel_of(perm, idx) : 0;
     const GT xo = Ro[p], yo = Ro[p + A.No], zo = Ro[p + 2 * A.No];
     const DT gp = valid ? ((const DT *) A.g)[frame * A.im_stride + p] : DT(0);
-    const __amdgpu_buffer_rsrc_t ry =
-        frame_rsrc((const DT *) A.Y + frame * A.y_stride, A.y_stride * (int64_t) sizeof(DT));
+    const DT *Yf = (const DT *) A.Y + frame * A.y_stride;
+    const __amdgpu_buffer_rsrc_t ry = ascan_rsrc(Yf, K, (int) sizeof(DT));
+    const __amdgpu_buffer_rsrc_t rf = frame_rsrc(Yf, A.y_stride * (int64_t) sizeof(DT));
+    const unsigned KB = (unsigned) K * (unsigned) sizeof(DT);
     GT gy_c = GT(0), gx_r = GT(0);
     if (TYPE != DAS_TFM) {
       gy_c = stripe_gap<GT>(yo, y_lo, y_hi);
@@ -1095,15 +1097,29 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
         const GT tt0 = tt;
         tt += dl;
         const DT wi = APOD ? at[i] : DT(1);
+        // Nearest-lower taps through the structured view of the frame (0
+        // outside [0, K) and for pairs that are not live); linear ones
+        // through the frame view with their own bounds tests (fewer
+        // registers there).
         DT y0[kRC], y1[kRC], f[kRC];
-        int k[kRC];
 #pragma unroll
         for (int jj = 0; jj < kRC; jj++) {
           const int j = min(j0 + jj, Lr - 1);
           const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
-          fetch_taps<DT, GT, INTERP>(ry, (unsigned) a * (unsigned) K, K,
-                                     (clamp_leg<GT>(tt, tmax) + clamp_leg<GT>(tr[jj], tmax)) * fs,
-                                     valid && j0 + jj < Lr, y0[jj], y1[jj], f[jj], k[jj]);
+          const GT u = (clamp_leg<GT>(tt, tmax) + clamp_leg<GT>(tr[jj], tmax)) * fs;
+          const bool live = valid && j0 + jj < Lr;
+          if (INTERP == DREAM_HIP_NEAREST) {
+            unsigned i0, i1;
+            int kk;
+            gather_index<DT, GT, INTERP, false>(u, live, i0, i1, f[jj], kk);
+            y0[jj] = sload<DT>(ry, i0, (unsigned) a * KB);
+            y1[jj] = y0[jj];
+            f[jj] = DT(0);
+          } else {
+            int kk;
+            fetch_taps<DT, GT, INTERP>(rf, (unsigned) a * (unsigned) K, K, u, live, y0[jj],
+                                       y1[jj], f[jj], kk);
+          }
         }
         DT zi = DT(0), zc = DT(0);
 #pragma unroll
