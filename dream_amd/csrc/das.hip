@@ -475,10 +475,46 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
           acc += APOD ? wi * part : part;
         }
       };
-      if (full && !diag)
+      if (full && !diag) {
         rows(std::true_type{});
-      else
-        rows(std::false_type{});
+        continue;
+      }
+      if constexpr (RECIP && TYPE == DAS_TFM && INTERP == DREAM_HIP_NEAREST && !APOD) {
+        if (diag && full) {
+          // A full diagonal block of the reciprocal path: row i0 + il is
+          // receiver j0 + il of the chunk (one array, the same expression,
+          // a zero delay: the same leg bit for bit), so the rows take their
+          // transmit legs from tr[] and, unrolled, issue only their live
+          // pairs jj >= il.  The sums are the generic rows' (their dead
+          // pairs add zeros).
+#pragma unroll
+          for (int il = 0; il < kRC; il++) {
+            const GT tt = tr[il];
+            const GT lo = (tt + trmin) * fs, hi = (tt + trmax) * fs;
+            const bool row_out = !valid || !(lo < (GT) K) || !(hi > GT(-1));
+            if (STATUS) oob |= !(lo > GT(-1)) || !(hi < (GT) K);
+            if (__all(row_out)) continue;
+            const unsigned abase = (unsigned) ((i0 + il) * Lr + j0) * KB;
+            unsigned o0[kRC];
+            DT y0[kRC];
+#pragma unroll
+            for (int jj = il; jj < kRC; jj++) {
+              unsigned o1;
+              DT f;
+              int k;
+              gather_index<DT, GT, INTERP, true>((tt + tr[jj]) * fs, true, o0[jj], o1, f, k);
+            }
+#pragma unroll
+            for (int jj = il; jj < kRC; jj++) y0[jj] = sload<DT>(rY, o0[jj], abase + jj * KB);
+            DT part = DT(0);
+#pragma unroll
+            for (int jj = il; jj < kRC; jj++) part += y0[jj];
+            acc += part;
+          }
+          continue;
+        }
+      }
+      rows(std::false_type{});
     }
   }
   return acc;
