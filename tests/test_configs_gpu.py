@@ -535,6 +535,40 @@ def test_unreadable_tensors_are_refused_before_launch(gpu):
     torch.cuda.synchronize()
 
 
+def test_overlapping_frames_are_refused(gpu):
+    """The forward reads each frame through a view of K x n_ascans samples
+    from its start: a frame stride below that (frames overlapping) is refused
+    before any launch, and a stride above it is honoured."""
+    import ctypes
+    from dream_amd import _lib
+    from dream_amd.torch_ops import DasPlan, _ptr, _stream
+    L, K, No = 8, 300, 64
+    G = np.c_[np.linspace(-4.0, 4.0, L), np.zeros(L), np.zeros(L)]
+    Ro = np.c_[np.zeros(No), np.zeros(No), np.linspace(2.0, 6.0, No)]
+    plan = DasPlan("tfm", K, Ro, G, G, precision="f32")
+    d, dtype = plan._delay(0.0)
+    stride = L * L * K
+    Y = torch.randn(2 * stride + 7, dtype=torch.float32, device="cuda")
+    Im = torch.zeros(2 * No, dtype=torch.float32, device="cuda")
+
+    def call(y_stride):
+        return _lib.lib().dream_hip_das_dev(
+            plan.h, _ptr(Y), _ptr(plan.Ro), _ptr(plan.Gt), _ptr(plan.Gr), 0.02, dtype, _ptr(d),
+            1500.0, None, None, 0, 2, y_stride, No, _ptr(Im), _lib.ERR_LEVEL["ignore"], None,
+            _stream())
+
+    assert call(stride - 1) != 0
+    assert "overlap" in _lib.lib().dream_hip_last_error().decode()
+    assert call(stride + 7) == 0
+    torch.cuda.synchronize()
+    ref0 = plan.forward(Y[:stride].view(L * L, K), 0.02, 0.0, 1500.0)
+    ref1 = plan.forward(Y[stride + 7:].view(L * L, K), 0.02, 0.0, 1500.0)
+    # (The one-frame and two-frame launches may be tuned to different
+    # shapes, which sum in another order.)
+    for got, ref in ((Im[:No], ref0), (Im[No:], ref1)):
+        assert torch.allclose(got, ref, rtol=1e-5, atol=1e-5 * float(ref.abs().max()))
+
+
 def test_frames_of_4gib_are_refused(gpu):
     """Gather offsets are 32-bit byte offsets inside one frame: a frame of
     4 GiB or more (FP64 TFM, 256 x 256 elements, K = 8192) is refused by the
