@@ -179,15 +179,27 @@ extern "C" __device__ double das_struct_load_f64(__amdgpu_buffer_rsrc_t r, int i
                                                  int soffset, int aux)
     __asm("llvm.amdgcn.struct.ptr.buffer.load.f64");
 
+// Four frames' samples of one (A-scan, sample) record (frame groups).
+typedef float frames4 __attribute__((ext_vector_type(4)));
+extern "C" __device__ frames4 das_struct_load_v4f32(__amdgpu_buffer_rsrc_t r, int index,
+                                                    int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v4f32");
+
 // Sample `index` of the A-scan at byte offset `soff` (0 outside [0, K)).
 template <typename DT>
 __device__ __forceinline__ DT sload(__amdgpu_buffer_rsrc_t r, unsigned index, unsigned soff)
 {
   if constexpr (std::is_same<DT, float>::value)
     return das_struct_load_f32(r, (int) index, 0, (int) soff, 0);
+  else if constexpr (std::is_same<DT, frames4>::value)
+    return das_struct_load_v4f32(r, (int) index, 0, (int) soff, 0);
   else
     return das_struct_load_f64(r, (int) index, 0, (int) soff, 0);
 }
+
+// What a lane sums: one frame's sample (DT), or four frames' (FR = 4).
+template <typename DT, int FR>
+using lane_t = typename std::conditional<FR == 4, frames4, DT>::type;
 
 template <typename DT>
 __device__ __forceinline__ DT bload(__amdgpu_buffer_rsrc_t r, unsigned byte_off);
@@ -296,8 +308,8 @@ constexpr int kMaxWaves = 8;     // waves per forward work-group (64 .. 512 thre
 // lane of the wave runs the same jobs).  `oob` is set when a pair of the
 // pixel fell outside [0, K) (only tracked when STATUS).
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD, bool STATUS,
-          bool RECIP = false>
-__device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__restrict__ perm,
+          bool RECIP = false, int FR = 1>
+__device__ __forceinline__ lane_t<DT, FR> fwd_tile_sum(const DasArgs &A, const int32_t *__restrict__ perm,
                                            int64_t idx, int64_t frame, int job0, int jstride,
                                            bool &oob)
 {
@@ -317,15 +329,18 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
   bool recip = false;
   if (RECIP && TYPE == DAS_TFM)
     recip = __builtin_amdgcn_readfirstlane((int) (delay_ms_of<GT>(gdel[0]) == GT(0))) != 0;
-  const DT *Yf = (const DT *) (recip ? A.Ysym : A.Y) + frame * A.y_stride;
-  const __amdgpu_buffer_rsrc_t rY = ascan_rsrc(Yf, K, (int) sizeof(DT));
-  const unsigned KB = Ku * (unsigned) sizeof(DT);          // bytes per A-scan
+  // One frame, or (FR = 4) frame group `frame` of the grouped copy.
+  using VT = lane_t<DT, FR>;
+  const VT *Yf = FR == 4 ? (const VT *) A.Yg + frame * A.yg_stride
+                         : (const VT *) (recip ? A.Ysym : A.Y) + frame * A.y_stride;
+  const __amdgpu_buffer_rsrc_t rY = ascan_rsrc(Yf, K, (int) sizeof(VT));
+  const unsigned KB = Ku * (unsigned) sizeof(VT);          // bytes per A-scan
   const GT *Gt = (const GT *) A.Gt;
   const GT *Gr = (const GT *) A.Gr;
   const DT *at = (const DT *) A.apod_t;
   const DT *ar = (const DT *) A.apod_r;
   const int Lt = A.Lt, Lr = A.Lr;
-  DT acc = DT(0);
+  VT acc = VT{};
 
   if (TYPE == DAS_SAFT) {
     const GT bnd = (GT) (0x1p-40 * (fabs(2.0 * (double) fs * (double) dl) + 1.0));
@@ -350,7 +365,8 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
       }
       // All loads of the block are issued before any is used.
       unsigned o0[kSB], o1[kSB];
-      DT f[kSB], y0[kSB], y1[kSB];
+      DT f[kSB];
+      VT y0[kSB], y1[kSB];
 #pragma unroll
       for (int q = 0; q < kSB; q++) {
         const bool live = n0 + q < Lt;
@@ -361,13 +377,13 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
 #pragma unroll
       for (int q = 0; q < kSB; q++) {
         const unsigned soff = (unsigned) min(n0 + q, Lt - 1) * KB;
-        y0[q] = sload<DT>(rY, o0[q], soff);
-        if (INTERP != DREAM_HIP_NEAREST) y1[q] = sload<DT>(rY, o1[q], soff);
+        y0[q] = sload<VT>(rY, o0[q], soff);
+        if (INTERP != DREAM_HIP_NEAREST) y1[q] = sload<VT>(rY, o1[q], soff);
       }
-      DT part = DT(0);
+      VT part = VT{};
 #pragma unroll
       for (int q = 0; q < kSB; q++) {
-        const DT v = INTERP == DREAM_HIP_NEAREST ? y0[q] : y0[q] + f[q] * (y1[q] - y0[q]);
+        const VT v = INTERP == DREAM_HIP_NEAREST ? y0[q] : y0[q] + f[q] * (y1[q] - y0[q]);
         part += APOD ? at[min(n0 + q, Lt - 1)] * v : v;
       }
       acc += part;
@@ -441,15 +457,17 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
           if (__all(row_out)) continue;
           const DT wi = APOD ? at[i] : DT(1);
           // Receivers in groups of kG (the whole chunk for nearest-lower
-          // sampling, half of it for linear interpolation): every load of a
-          // group is issued before any is used.  Receivers past Lr (in the
+          // sampling, half of it for linear interpolation, half that again
+          // for frame groups): every load of a group is issued before any
+          // is used.  Receivers past Lr (in the
           // last chunk) read 0.
-          constexpr int kG = INTERP == DREAM_HIP_NEAREST ? kRC : kRC / 2;
-          DT part = DT(0);
+          constexpr int kG = (INTERP == DREAM_HIP_NEAREST ? kRC : kRC / 2) / (FR == 4 ? 2 : 1);
+          VT part = VT{};
 #pragma unroll
           for (int g0 = 0; g0 < kRC; g0 += kG) {
             unsigned o0[kG], o1[kG];
-            DT f[kG], y0[kG], y1[kG];
+            DT f[kG];
+            VT y0[kG], y1[kG];
 #pragma unroll
             for (int jj = 0; jj < kG; jj++) {
               const bool live = (full || j0 + g0 + jj < Lr) && (!diag || j0 + g0 + jj >= i);
@@ -462,12 +480,12 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
               const int j = min(j0 + g0 + jj, Lr - 1);
               const int a = (TYPE == DAS_RCA_ROWTX) ? j * Lt + i : i * Lr + j;
               const unsigned soff = (unsigned) a * KB;
-              y0[jj] = sload<DT>(rY, o0[jj], soff);
-              if (INTERP != DREAM_HIP_NEAREST) y1[jj] = sload<DT>(rY, o1[jj], soff);
+              y0[jj] = sload<VT>(rY, o0[jj], soff);
+              if (INTERP != DREAM_HIP_NEAREST) y1[jj] = sload<VT>(rY, o1[jj], soff);
             }
 #pragma unroll
             for (int jj = 0; jj < kG; jj++) {
-              const DT v =
+              const VT v =
                   INTERP == DREAM_HIP_NEAREST ? y0[jj] : y0[jj] + f[jj] * (y1[jj] - y0[jj]);
               part += APOD ? ar[min(j0 + g0 + jj, Lr - 1)] * v : v;
             }
@@ -496,7 +514,7 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
             if (__all(row_out)) continue;
             const unsigned abase = (unsigned) ((i0 + il) * Lr + j0) * KB;
             unsigned o0[kRC];
-            DT y0[kRC];
+            VT y0[kRC];
 #pragma unroll
             for (int jj = il; jj < kRC; jj++) {
               unsigned o1;
@@ -505,8 +523,8 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
               gather_index<DT, GT, INTERP, true>((tt + tr[jj]) * fs, true, o0[jj], o1, f, k);
             }
 #pragma unroll
-            for (int jj = il; jj < kRC; jj++) y0[jj] = sload<DT>(rY, o0[jj], abase + jj * KB);
-            DT part = DT(0);
+            for (int jj = il; jj < kRC; jj++) y0[jj] = sload<VT>(rY, o0[jj], abase + jj * KB);
+            VT part = VT{};
 #pragma unroll
             for (int jj = il; jj < kRC; jj++) part += y0[jj];
             acc += part;
@@ -521,11 +539,12 @@ __device__ __forceinline__ DT fwd_tile_sum(const DasArgs &A, const int32_t *__re
 }
 
 template <typename DT, typename GT, int TYPE, int INTERP, bool APOD, bool STATUS,
-          bool RECIP = false>
+          bool RECIP = false, int FR = 1>
 __global__ void __launch_bounds__(64 * kMaxWaves)
 das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__restrict__ partial)
 {
-  __shared__ DT red[kMaxWaves][64];
+  using VT = lane_t<DT, FR>;
+  __shared__ VT red[kMaxWaves][64];
   const int nwaves = blockDim.x >> 6;
   // The wave index is uniform across the wave; say so, so that job, element
   // and A-scan indices live in SGPRs.
@@ -537,16 +556,28 @@ das_fwd_kernel(DasArgs A, const int32_t *__restrict__ perm, int split, DT *__res
   const bool valid = idx < A.No;
   const int64_t p = valid ? pixel_of(perm, idx) : 0;
   bool oob = false;
-  const DT acc = fwd_tile_sum<DT, GT, TYPE, INTERP, APOD, STATUS, RECIP>(
+  const VT acc = fwd_tile_sum<DT, GT, TYPE, INTERP, APOD, STATUS, RECIP, FR>(
       A, perm, idx, frame, sidx * nwaves + wave, nwaves * split, oob);
   // Sum the waves' partial sums in a fixed order.
   red[wave][lane] = acc;
   __syncthreads();
   if (wave == 0 && valid) {
-    DT sum = red[0][lane];
+    VT sum = red[0][lane];
     for (int w = 1; w < nwaves; w++) sum += red[w][lane];
-    if (split == 1) ((DT *) A.Im)[frame * A.im_stride + p] = sum;
-    else partial[((int64_t) sidx * A.n_frames + frame) * A.No + p] = sum;
+    if constexpr (FR == 1) {
+      if (split == 1) ((DT *) A.Im)[frame * A.im_stride + p] = sum;
+      else partial[((int64_t) sidx * A.n_frames + frame) * A.No + p] = sum;
+    } else {
+      // Group `frame` holds frames FR * frame .. FR * frame + FR - 1.
+#pragma unroll
+      for (int q = 0; q < FR; q++) {
+        const int64_t fr = FR * frame + q;
+        if (fr < A.n_frames) {
+          if (split == 1) ((DT *) A.Im)[fr * A.im_stride + p] = sum[q];
+          else partial[((int64_t) sidx * A.n_frames + fr) * A.No + p] = sum[q];
+        }
+      }
+    }
   }
   if (STATUS) {
     // Every writer of a call stores the same value, the call's epoch, so a
@@ -655,6 +686,27 @@ das_symmetrize_kernel(const DT *__restrict__ Y, DT *__restrict__ Ys, int L, int6
     for (int64_t k = threadIdx.x; k < K; k += step) o[k] = a[k];
   } else {
     for (int64_t k = threadIdx.x; k < K; k += step) o[k] = a[k] + b[k];
+  }
+}
+
+// Yg[g][e] = the samples e (of the n_ascans K of one frame) of frames
+// 4g .. 4g+3 as one 16-byte record, 0 for frames past the last: the frame
+// groups of a multi-frame FP32 two-leg forward.  Reads are coalesced per
+// frame, writes per record.
+__global__ void __launch_bounds__(256)
+das_group_frames_kernel(const float *__restrict__ Y, int64_t y_stride, int64_t E,
+                        int64_t n_frames, frames4 *__restrict__ Yg)
+{
+  const int64_t g = blockIdx.y;
+  const int64_t step = (int64_t) gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; e < E; e += step) {
+    frames4 v;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int64_t f = 4 * g + q;
+      v[q] = f < n_frames ? Y[f * y_stride + e] : 0.0f;
+    }
+    Yg[g * E + e] = v;
   }
 }
 
@@ -1070,11 +1122,15 @@ das_adj_region_kernel(DasArgs A, const int32_t *__restrict__ perm, int n_gj, int
 // Pixel-major adjoint for two-leg modes without gY: accumulates
 // z_t[i] = sum_p g_p sum_j w_j v(p,i,j), z_r[j] = sum_p g_p sum_i w_i v(p,i,j)
 // and z_cp, in LDS per work-group, flushed once.  Grid-stride over
-// (frame, 256-pixel tile); gathers as in the forward kernel.
-template <typename DT, typename GT, int TYPE, int INTERP, bool APOD>
+// (frame, 256-pixel tile); gathers as in the forward kernel.  FR = 4: over
+// (frame group, tile), each gather bringing four frames' samples, which
+// the lane weights with its four g values (nearest-lower sampling).
+template <typename DT, typename GT, int TYPE, int INTERP, bool APOD, int FR = 1>
 __global__ void __launch_bounds__(kThreads)
 das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tiles)
 {
+  static_assert(FR == 1 || (INTERP == DREAM_HIP_NEAREST && std::is_same<DT, float>::value),
+                "frame groups: FP32, nearest-lower sampling");
   extern __shared__ __align__(16) unsigned char smem[];
   double *zt = reinterpret_cast<double *>(smem);
   double *zr = zt + A.Lt;
@@ -1102,11 +1158,25 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
     const bool valid = idx < A.No;
     const int64_t p = valid ? pixel_of(perm, idx) : 0;
     const GT xo = Ro[p], yo = Ro[p + A.No], zo = Ro[p + 2 * A.No];
-    const DT gp = valid ? ((const DT *) A.g)[frame * A.im_stride + p] : DT(0);
-    const DT *Yf = (const DT *) A.Y + frame * A.y_stride;
-    const __amdgpu_buffer_rsrc_t ry = ascan_rsrc(Yf, K, (int) sizeof(DT));
+    // `frame` is the frame group when FR = 4: its g values in gp4, and gp
+    // (the weight of the per-row sums) 1.
+    const DT *gv = (const DT *) A.g;
+    DT gp = DT(0);
+    frames4 gp4 = frames4{};
+    if constexpr (FR == 4) {
+      gp = DT(1);
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        gp4[q] = valid && 4 * frame + q < A.n_frames ? gv[(4 * frame + q) * A.im_stride + p] : 0.0f;
+    } else {
+      gp = valid ? gv[frame * A.im_stride + p] : DT(0);
+    }
+    const DT *Yf = (const DT *) A.Y + (FR == 4 ? 0 : frame * A.y_stride);
+    const __amdgpu_buffer_rsrc_t ry =
+        FR == 4 ? ascan_rsrc((const frames4 *) A.Yg + frame * A.yg_stride, K, 16)
+                : ascan_rsrc(Yf, K, (int) sizeof(DT));
     const __amdgpu_buffer_rsrc_t rf = frame_rsrc(Yf, A.y_stride * (int64_t) sizeof(DT));
-    const unsigned KB = (unsigned) K * (unsigned) sizeof(DT);
+    const unsigned KB = (unsigned) K * (unsigned) (FR == 4 ? 16 : sizeof(DT));
     GT gy_c = GT(0), gx_r = GT(0);
     if (TYPE != DAS_TFM) {
       gy_c = stripe_gap<GT>(yo, y_lo, y_hi);
@@ -1148,7 +1218,12 @@ das_adj_pixel_kernel(DasArgs A, const int32_t *__restrict__ perm, int64_t n_tile
             unsigned i0, i1;
             int kk;
             gather_index<DT, GT, INTERP, false>(u, live, i0, i1, f[jj], kk);
-            y0[jj] = sload<DT>(ry, i0, (unsigned) a * KB);
+            if constexpr (FR == 4) {
+              const frames4 v = sload<frames4>(ry, i0, (unsigned) a * KB);
+              y0[jj] = gp4[0] * v[0] + gp4[1] * v[1] + gp4[2] * v[2] + gp4[3] * v[3];
+            } else {
+              y0[jj] = sload<DT>(ry, i0, (unsigned) a * KB);
+            }
             y1[jj] = y0[jj];
             f[jj] = DT(0);
           } else {
@@ -1258,6 +1333,17 @@ bool rca_extents_in_bounds(int64_t Lt, int64_t Lr)
 
 // DREAM_HIP_RECIPROCAL=0 switches the reciprocal TFM forward off (the full
 // pair sum runs), for comparisons.
+// Frame groups (DasArgs::Yg) for multi-frame FP32 two-leg forwards;
+// DREAM_HIP_FRAME_GROUPS=0 gives one frame per launch row (for comparisons).
+bool frame_groups_enabled()
+{
+  static const bool on = [] {
+    const char *e = std::getenv("DREAM_HIP_FRAME_GROUPS");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 bool recip_enabled()
 {
   static const bool on = [] {
@@ -1405,9 +1491,13 @@ void fwd_launch_shape(const DasArgs &a0, FwdShape sh, hipStream_t s, ChunkGate *
     sh.split = 1;
   dim3 block((unsigned) (64 * sh.waves));
   const int64_t tiles = (a0.No + 63) / 64;
-  // A chunk is a range of tiles of some frames; with split partial sums its
-  // pixels are reduced right after it, so every launched chunk's pixels are
-  // written when a later chunk is cancelled.
+  // Launch rows: frames, or groups of four frames (a0.Yg, FP32 two-leg).
+  constexpr bool kGroups = std::is_same<DT, float>::value && TYPE != DAS_SAFT;
+  const bool groups = kGroups && a0.Yg != nullptr;
+  const int64_t n_rows = groups ? (a0.n_frames + 3) / 4 : a0.n_frames;
+  // A chunk is a range of tiles of some launch rows; with split partial
+  // sums its pixels are reduced right after it, so every launched chunk's
+  // pixels are written when a later chunk is cancelled.
   auto launch = [&](int64_t b0, int64_t nb, int64_t f0, int64_t nf) {
     DasArgs a = a0;
     a.block0 = b0;
@@ -1416,7 +1506,16 @@ void fwd_launch_shape(const DasArgs &a0, FwdShape sh, hipStream_t s, ChunkGate *
     // The reciprocal TFM kernel is its own instance (no apodization), so the
     // other kernels keep their registers.
     constexpr bool kRecip = TYPE == DAS_TFM && !APOD;
-    if (kRecip && a.recip) {
+    if (groups) {
+      if constexpr (kGroups) {
+        if (a.status)
+          hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, true, false, 4>), grid,
+                             block, 0, s, a, a.perm, sh.split, (DT *) a.scratch);
+        else
+          hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, false, false, 4>), grid,
+                             block, 0, s, a, a.perm, sh.split, (DT *) a.scratch);
+      }
+    } else if (kRecip && a.recip) {
       if (a.status)
         hipLaunchKernelGGL((das_fwd_kernel<DT, GT, TYPE, INTERP, APOD, true, kRecip>), grid,
                            block, 0, s, a, a.perm, sh.split, (DT *) a.scratch);
@@ -1433,14 +1532,17 @@ void fwd_launch_shape(const DasArgs &a0, FwdShape sh, hipStream_t s, ChunkGate *
     DREAM_HIP_CHECK(hipGetLastError());
     if (sh.split > 1) {
       const int64_t n_idx = std::min<int64_t>(nb * 64, a.No - b0 * 64);
-      const int64_t n = n_idx * nf;
+      const int64_t fr0 = groups ? 4 * f0 : f0;
+      const int64_t frn = groups ? std::min<int64_t>(4 * (f0 + nf), a.n_frames) - fr0 : nf;
+      const int64_t n = n_idx * frn;
       hipLaunchKernelGGL(das_reduce_kernel<DT>, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0,
-                         s, (const DT *) a.scratch, sh.split, a.No, a.n_frames, f0, nf, b0 * 64,
+                         s, (const DT *) a.scratch, sh.split, a.No, a.n_frames, fr0, frn, b0 * 64,
                          n_idx, a.perm, a.im_stride, (DT *) a.Im);
       DREAM_HIP_CHECK(hipGetLastError());
     }
   };
-  run_chunked(tiles, a0.n_frames, 64 * pairs_per_pixel(TYPE, a0.Lt, a0.Lr), gate, launch);
+  run_chunked(tiles, n_rows, 64 * pairs_per_pixel(TYPE, a0.Lt, a0.Lr) * (groups ? 4 : 1), gate,
+              launch);
 }
 
 // The heuristic shape, or the autotuner's pick for this shape class (with
@@ -1453,10 +1555,10 @@ void fwd_launch(const DasArgs &a, hipStream_t s, ChunkGate *gate)
   int pick = 0;
   if (cand.size() > 1) {
     char key[192];
-    std::snprintf(key, sizeof key, "das_fwd|%d|%d|%d|%d|%d|%lld|%d|%d|%lld|%lld|%d|%d", TYPE,
+    std::snprintf(key, sizeof key, "das_fwd|%d|%d|%d|%d|%d|%lld|%d|%d|%lld|%lld|%d|%d|%d", TYPE,
                   (int) sizeof(DT), (int) sizeof(GT), INTERP, (int) APOD, (long long) a.No, a.Lt,
                   a.Lr, (long long) a.K, (long long) a.n_frames, (int) a.recip,
-                  (int) (a.status != nullptr));
+                  (int) (a.status != nullptr), (int) (a.Yg != nullptr));
     const int t = autotune(key, (int) cand.size(),
                            [&](int i, hipStream_t st) { fwd_launch_shape<DT, GT, TYPE, INTERP, APOD>(a, cand[(size_t) i], st); },
                            s);
@@ -1557,16 +1659,27 @@ void adj_launch(const DasArgs &a, const int32_t *perm, bool want_gy, hipStream_t
     }
     run(subs_cand[pick], false, s, gate);
   } else {
-    // Grid-stride over (frame, 256-pixel tile); a chunk is a range of tiles.
-    const int64_t tiles = ((a.No + kThreads - 1) / kThreads) * a.n_frames;
+    // Grid-stride over (frame or frame group, 256-pixel tile); a chunk is a
+    // range of tiles.
+    constexpr bool kGroups = std::is_same<DT, float>::value && INTERP == DREAM_HIP_NEAREST;
+    const bool groups = kGroups && a.Yg != nullptr;
+    const int64_t rows = groups ? (a.n_frames + 3) / 4 : a.n_frames;
+    const int64_t tiles = ((a.No + kThreads - 1) / kThreads) * rows;
     const size_t lds = (size_t) (a.Lt + a.Lr) * sizeof(double);
-    run_chunked(tiles, 1, (int64_t) kThreads * pairs_per_pixel(TYPE, a.Lt, a.Lr), gate,
+    run_chunked(tiles, 1,
+                (int64_t) kThreads * pairs_per_pixel(TYPE, a.Lt, a.Lr) * (groups ? 4 : 1), gate,
                 [&](int64_t b0, int64_t nb, int64_t, int64_t) {
                   DasArgs c = a;
                   c.block0 = b0;
                   const int grid = (int) std::min<int64_t>(nb, (int64_t) cu_count() * 4);
-                  hipLaunchKernelGGL((das_adj_pixel_kernel<DT, GT, TYPE, INTERP, APOD>),
-                                     dim3(grid), dim3(kThreads), lds, s, c, perm, b0 + nb);
+                  if (groups) {
+                    if constexpr (kGroups)
+                      hipLaunchKernelGGL((das_adj_pixel_kernel<DT, GT, TYPE, INTERP, APOD, 4>),
+                                         dim3(grid), dim3(kThreads), lds, s, c, perm, b0 + nb);
+                  } else {
+                    hipLaunchKernelGGL((das_adj_pixel_kernel<DT, GT, TYPE, INTERP, APOD>),
+                                       dim3(grid), dim3(kThreads), lds, s, c, perm, b0 + nb);
+                  }
                   DREAM_HIP_CHECK(hipGetLastError());
                 });
   }
@@ -1796,6 +1909,7 @@ DasHandle::~DasHandle()
 {
   drop_timing();
   (void) hipFree(d_sym_);
+  (void) hipFree(d_grp_);
   (void) hipFree(d_status_);
   (void) hipHostFree(h_status_);
   (void) hipFree(d_buf_);
@@ -1908,6 +2022,34 @@ void DasHandle::fill_args(DasArgs &a, const void *Y, const void *Ro, const void 
   a.scratch_bytes = d_scratch_bytes_;
 }
 
+// Several FP32 frames of a two-leg array: the frames in groups of four
+// (a.Yg), so that one gather serves four frames -- a copy of Y in the
+// handle, made on stream s; one frame per launch row if it cannot be had.
+void DasHandle::group_frames(DasArgs &a, const void *Y, int64_t y_stride, int64_t n_frames,
+                             hipStream_t s)
+{
+  const int64_t E = (type_ == DAS_SAFT ? Lt_ : Lt_ * Lr_) * K_;
+  if (prec_ != DREAM_HIP_F32 || type_ == DAS_SAFT || n_frames < 2 || !frame_groups_enabled() ||
+      (uint64_t) E * 16 >= 0xFFFFFFF0ull)
+    return;
+  const int64_t G = (n_frames + 3) / 4;
+  const size_t bytes = (size_t) (G * E) * 16;
+  if (bytes > d_grp_bytes_) {
+    (void) hipFree(d_grp_);
+    d_grp_ = nullptr;
+    d_grp_bytes_ = 0;
+    if (hipMalloc(&d_grp_, bytes) == hipSuccess) d_grp_bytes_ = bytes;
+    else (void) hipGetLastError();
+  }
+  if (!d_grp_) return;
+  const unsigned bx = (unsigned) std::min<int64_t>((E + 255) / 256, (int64_t) 1 << 20);
+  hipLaunchKernelGGL(das_group_frames_kernel, dim3(bx, (unsigned) G), dim3(256), 0, s,
+                     (const float *) Y, y_stride, E, n_frames, (frames4 *) d_grp_);
+  DREAM_HIP_CHECK(hipGetLastError());
+  a.Yg = d_grp_;
+  a.yg_stride = E;
+}
+
 int DasHandle::run_dev(const void *Y, const void *Ro, const void *Gt, const void *Gr, double dt,
                        int delay_type, const void *delay, double cp, const void *apod_t,
                        const void *apod_r, int interp, int64_t n_frames, int64_t y_stride,
@@ -1973,6 +2115,7 @@ int DasHandle::run_dev(const void *Y, const void *Ro, const void *Gt, const void
     a.Ysym = d_sym_;
     a.recip = 1;
   }
+  if (No_ > 0 && !a.recip) group_frames(a, Y, y_stride, n_frames, s);
   if (timed) DREAM_HIP_CHECK(hipEventRecord(tev_[(size_t) 3 * t_n_ + 1], s));
   ChunkGate gate(s);
   if (No_ > 0) launch_das_forward(a, type_, prec_, interp, apod, s, &gate);
@@ -2110,6 +2253,10 @@ void DasHandle::adjoint_dev(const void *Y, const void *Ro, const void *Gt, const
   const bool channel = type_ == DAS_SAFT || gY != nullptr;
   a.z_t = channel ? (void *) Z : (void *) zt;
   a.z_r = zr;
+  // The pixel kernel's nearest-lower taps read frames in groups of four
+  // (not in the allocation-free call).
+  if (No_ > 0 && !channel && !fixed_z && interp == DREAM_HIP_NEAREST)
+    group_frames(a, Y, y_stride, n_frames, s);
   ChunkGate gate(s);
   if (No_ > 0) {
     if (fixed_z) {

@@ -48,6 +48,13 @@ struct DasArgs {
   // expression, added commutatively).
   const void *Ysym;
   int32_t recip;
+  // Frames in groups of four (FP32 two-leg forwards of several frames):
+  // Yg holds, per group, one 16-byte record of the four frames' samples
+  // per (A-scan, sample) -- group g at Yg + g * yg_stride records, frames
+  // past the last read as 0 -- so one gather serves four frames.  Null: one
+  // frame per launch row.
+  const void *Yg;
+  int64_t yg_stride;
 };
 
 // Bytes of forward partial-sum scratch a launch of this shape uses.
@@ -120,6 +127,8 @@ private:
   size_t d_scratch_bytes_ = 0;
   void *d_sym_ = nullptr;        // reciprocal TFM: symmetrized A-scans (Ysym)
   size_t d_sym_bytes_ = 0;
+  void *d_grp_ = nullptr;        // frames in groups of four (Yg)
+  size_t d_grp_bytes_ = 0;
   int32_t *d_perm_ = nullptr;    // pixel order (No entries) or null
 public:
   // Set (perm != null, n == No) or clear (perm == null) the pixel order; perm
@@ -133,6 +142,8 @@ public:
   void read_timing(int *n, double *prep_ms, double *forward_ms);
 private:
   void ensure_scratch(size_t bytes);
+  void group_frames(DasArgs &a, const void *Y, int64_t y_stride, int64_t n_frames,
+                    hipStream_t s);
   void drop_timing();
   std::vector<hipEvent_t> tev_;  // 3 per timed call
   int t_cap_ = 0, t_n_ = 0;

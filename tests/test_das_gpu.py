@@ -290,6 +290,40 @@ def test_adjoint_frames_equal_single_calls(gpu, method, want_gy):
         assert abs(float(gcp) - scp) <= 1e-10 * abs(scp)
 
 
+@pytest.mark.parametrize("method", ["tfm", "rca_coltx", "rca_rowtx"])
+def test_frame_groups_match_single_frames(gpu, method):
+    """FP32 calls of several frames read them in groups of four (one
+    gather serving four frames; five frames: a full group and a group with
+    three padding frames).  Each frame's image is its single-frame image,
+    and the nearest-lower apodization gradients are the sums of the single
+    frames' (to FP32 rounding: the frames are summed in another order)."""
+    from dream_amd.torch_ops import DasPlan
+    Y, Gt, Gr, Ro, rng = _setup(method, seed=11, No=1300)
+    if method == "tfm":
+        Gr = Gr + np.array([0.05, 0.0, 0.0])        # two arrays: not the reciprocal path
+    F = 5
+    Yf = torch.tensor(np.stack([Y.T * (f + 1) + f for f in range(F)]), dtype=torch.float32,
+                      device="cuda").contiguous()
+    g = torch.tensor(rng.standard_normal((F, Ro.shape[0])), dtype=torch.float32,
+                     device="cuda").contiguous()
+    plan = DasPlan(method, Y.shape[0], Ro, Gt, Gr, precision="f32")
+    at = torch.tensor(rng.uniform(0.5, 1.5, Gt.shape[0]), dtype=torch.float32, device="cuda")
+    ar = torch.tensor(rng.uniform(0.5, 1.5, Gr.shape[0]), dtype=torch.float32, device="cuda")
+    Im = plan.forward(Yf, 0.02, 0.1, 1500.0)
+    _, gat, gar, _ = plan.adjoint(Yf, g, 0.02, 0.1, 1500.0, at, ar, "nearest", want_gy=False,
+                                  want_cp=False)
+    sat, sar = 0.0, 0.0
+    for f in range(F):
+        single = plan.forward(Yf[f].contiguous(), 0.02, 0.1, 1500.0).cpu().numpy()
+        assert rel_l2(Im[f].cpu().numpy(), single) <= 1e-6, f
+        _, a1, r1, _ = plan.adjoint(Yf[f].contiguous(), g[f].contiguous(), 0.02, 0.1, 1500.0,
+                                    at, ar, "nearest", want_gy=False, want_cp=False)
+        sat = sat + a1.double().cpu().numpy()
+        sar = sar + r1.double().cpu().numpy()
+    assert rel_l2(gat.double().cpu().numpy(), sat) <= 1e-5
+    assert rel_l2(gar.double().cpu().numpy(), sar) <= 1e-5
+
+
 def test_autograd_function(gpu, oracle):
     from dream_amd.torch_ops import DasPlan, das_diff
     method = "saft"
