@@ -360,7 +360,10 @@ int dream_hip_das(void *handle, const void *Y, const void *Ro, const void *Gt,
  * call is asynchronous and dream_hip_das_status reads the code later; with
  * sir_err it waits for its stream and writes the code.  Calls on one handle
  * must be ordered (one stream, or streams synchronised between the calls):
- * they share the handle's work buffers and status word. */
+ * they share the handle's work buffers and status word.  Several FP32
+ * frames of a two-leg (TFM with two arrays, RCA) call are first copied into
+ * groups of four frames (device memory of the handle, Y's size rounded up
+ * to four frames), so that one gather serves four frames. */
 int dream_hip_das_dev(void *handle, const void *Y, const void *Ro, const void *Gt,
                       const void *Gr, double dt, int delay_type, const void *delay,
                       double cp, const void *apod_t, const void *apod_r, int interp,
