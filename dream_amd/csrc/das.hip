@@ -2035,6 +2035,10 @@ void DasHandle::group_frames(DasArgs &a, const void *Y, int64_t y_stride, int64_
   const int64_t G = (n_frames + 3) / 4;
   const size_t bytes = (size_t) (G * E) * 16;
   if (bytes > d_grp_bytes_) {
+    // No allocation while the stream is being captured into a graph: the
+    // captured call reads one frame per gather.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
     (void) hipFree(d_grp_);
     d_grp_ = nullptr;
     d_grp_bytes_ = 0;
