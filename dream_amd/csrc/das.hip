@@ -635,6 +635,32 @@ __global__ void das_reduce_kernel(const DT *__restrict__ partial, int split, int
   Im[f * im_stride + p] = sum;
 }
 
+// The same sums when a launch covers every pixel of its frames and the
+// image rows are contiguous (im_stride == No): out[i] = sum over s of
+// partial[s * stride + i], four consecutive values per thread, no pixel
+// order to look up (n4 groups of four; partial and out offset by the
+// launch's first frame, aligned to four values).
+template <typename DT>
+__global__ void das_reduce_flat_kernel(const DT *__restrict__ partial, int split, int64_t stride,
+                                       int64_t n4, DT *__restrict__ out)
+{
+  typedef DT v4 __attribute__((ext_vector_type(4)));
+  const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  const v4 *P = (const v4 *) partial + i;
+  const int64_t st = stride / 4;
+  v4 sum = P[0];
+  int t = 1;
+  for (; t + 3 <= split; t += 3) {
+    const v4 a = P[t * st], b = P[(t + 1) * st], c = P[(t + 2) * st];
+    sum += a;
+    sum += b;
+    sum += c;
+  }
+  for (; t < split; t++) sum += P[t * st];
+  ((v4 *) out)[i] = sum;
+}
+
 // Ysym[f][i L + j] = Y[f][i L + j] + Y[f][j L + i] (i < j), Y[f][i L + i]
 // (i == j), for the reciprocal TFM forward; nothing when the scalar delay
 // read on the device is not zero (the forward then reads Y).  One
@@ -1535,9 +1561,17 @@ void fwd_launch_shape(const DasArgs &a0, FwdShape sh, hipStream_t s, ChunkGate *
       const int64_t fr0 = groups ? 4 * f0 : f0;
       const int64_t frn = groups ? std::min<int64_t>(4 * (f0 + nf), a.n_frames) - fr0 : nf;
       const int64_t n = n_idx * frn;
-      hipLaunchKernelGGL(das_reduce_kernel<DT>, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0,
-                         s, (const DT *) a.scratch, sh.split, a.No, a.n_frames, fr0, frn, b0 * 64,
-                         n_idx, a.perm, a.im_stride, (DT *) a.Im);
+      DT *out = (DT *) a.Im + fr0 * a.No;
+      if (b0 == 0 && n_idx == a.No && a.im_stride == a.No && a.No % 4 == 0 &&
+          (uintptr_t) out % (4 * sizeof(DT)) == 0) {
+        hipLaunchKernelGGL(das_reduce_flat_kernel<DT>, dim3((unsigned) ((n / 4 + 255) / 256)),
+                           dim3(256), 0, s, (const DT *) a.scratch + fr0 * a.No, sh.split,
+                           a.n_frames * a.No, n / 4, out);
+      } else {
+        hipLaunchKernelGGL(das_reduce_kernel<DT>, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0,
+                           s, (const DT *) a.scratch, sh.split, a.No, a.n_frames, fr0, frn, b0 * 64,
+                           n_idx, a.perm, a.im_stride, (DT *) a.Im);
+      }
       DREAM_HIP_CHECK(hipGetLastError());
     }
   };

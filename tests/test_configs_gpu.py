@@ -569,6 +569,55 @@ def test_overlapping_frames_are_refused(gpu):
         assert torch.allclose(got, ref, rtol=1e-5, atol=1e-5 * float(ref.abs().max()))
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_split_sums_of_flat_and_strided_images_agree(gpu, prec):
+    """A split forward launch (work-groups sharing a tile, partial sums added
+    after it) adds four pixels at a time when the image rows are contiguous
+    and pixel by pixel, in visiting order, when they are not: the two give
+    the same two-frame image bit for bit.  128 x 128 pairs make 64 jobs per
+    tile, so the untuned shape splits each tile over eight work-groups."""
+    from dream_amd import _lib
+    from dream_amd.torch_ops import DasPlan, _ptr, _stream
+    L, K, F = 128, 256, 2
+    x = np.linspace(-8.0, 8.0, L)
+    Gt = np.c_[x, np.zeros(L), np.zeros(L)]
+    Gr = np.c_[x, np.zeros(L), np.full(L, 0.01)]
+    X, Z = np.meshgrid(np.linspace(-6.0, 6.0, 64), np.linspace(2.0, 5.0, 64), indexing="ij")
+    Ro = np.c_[X.ravel(), np.zeros(X.size), Z.ravel()]
+    No = Ro.shape[0]
+    dtype = torch.float64 if prec == "f64" else torch.float32
+    plan = DasPlan("tfm", K, Ro, Gt, Gr, precision=prec)
+    d, dtype_code = plan._delay(0.0)
+    torch.manual_seed(3)
+    Y = torch.randn(F * L * L * K, dtype=dtype, device="cuda")
+
+    def run(im_stride):
+        Im = torch.full(((F - 1) * im_stride + No,), 7.0, dtype=dtype, device="cuda")
+        rc = _lib.lib().dream_hip_das_dev(
+            plan.h, _ptr(Y), _ptr(plan.Ro), _ptr(plan.Gt), _ptr(plan.Gr), 0.02, dtype_code,
+            _ptr(d), 1500.0, None, None, 0, F, 0, im_stride, _ptr(Im),
+            _lib.ERR_LEVEL["ignore"], None, _stream())
+        assert rc == 0, _lib.lib().dream_hip_last_error().decode()
+        torch.cuda.synchronize()
+        return [Im[f * im_stride:f * im_stride + No] for f in range(F)]
+
+    gpu.dream_hip_set_autotune(0)
+    try:
+        flat = run(No)
+        strided = run(No + 4)
+    finally:
+        gpu.dream_hip_set_autotune(1)
+    for a, b in zip(flat, strided):
+        assert torch.equal(a, b)
+        assert float(a.abs().max()) > 0.0
+    # Against single-frame calls of the plan (own launch shape: to rounding).
+    for f in range(F):
+        ref = plan.forward(Y[f * L * L * K:(f + 1) * L * L * K].view(L * L, K), 0.02, 0.0,
+                           1500.0)
+        assert rel_l2(flat[f].double().cpu().numpy(), ref.double().cpu().numpy()) <= (
+            1e-13 if prec == "f64" else 1e-5)
+
+
 def test_frames_of_4gib_are_refused(gpu):
     """Gather offsets are 32-bit byte offsets inside one frame: a frame of
     4 GiB or more (FP64 TFM, 256 x 256 elements, K = 8192) is refused by the
